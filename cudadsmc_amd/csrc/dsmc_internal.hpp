@@ -1,0 +1,113 @@
+// dsmc_internal.hpp -- host-side internals shared by the library's
+// translation units: launch bookkeeping (optional per-kernel hipEvent timing),
+// error propagation, and the device-primitive launchers (radix sort, scan,
+// compaction) that both numerical forms use.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace dsmc {
+
+// Thrown inside the library only; every extern "C" entry point catches it and
+// converts it into a dsmc_status plus a message.
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+void hip_check(hipError_t e, const char *what, const char *file, int line);
+#define DSMC_HIP(call) ::dsmc::hip_check((call), #call, __FILE__, __LINE__)
+
+// Per-kernel timing with hipEvents on the launch stream.
+class Profiler {
+public:
+    bool enabled = false;
+    void begin(hipStream_t s, const char *name, double bytes);
+    void end(hipStream_t s);
+    // Resolve pending events (synchronises) and fold them into the totals.
+    void collect();
+    void reset();
+    struct Stat { long long launches = 0; double total_ms = 0.0; double bytes = 0.0; };
+    const std::map<std::string, Stat> &stats() const { return stats_; }
+    ~Profiler();
+
+private:
+    struct Rec { std::string name; double bytes; hipEvent_t b, e; };
+    std::vector<Rec> pending_;
+    std::vector<hipEvent_t> pool_;
+    std::map<std::string, Stat> stats_;
+    hipEvent_t take();
+};
+
+// Launch `kernel` with optional profiling.  `bytes` is the algorithmic byte
+// count of this launch (used only for reporting).
+#define DSMC_LAUNCH(prof, name, bytes, kernel, grid, block, stream, ...)            \
+    do {                                                                            \
+        ::dsmc::Profiler *p_ = (prof);                                              \
+        if (p_ && p_->enabled) p_->begin((stream), (name), (double)(bytes));        \
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, (stream), __VA_ARGS__); \
+        DSMC_HIP(hipGetLastError());                                                \
+        if (p_ && p_->enabled) p_->end((stream));                                   \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device primitives (primitives.hip).
+// ---------------------------------------------------------------------------
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortBlock * kSortItems;
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// Scratch for a radix sort of up to n_cap pairs.
+struct SortScratch {
+    int n_cap = 0;
+    int nb = 0;                   // blocks per pass
+    uint32_t *hist = nullptr;     // kRadix * nb
+    uint32_t *totals = nullptr;   // kRadix
+};
+size_t sort_scratch_bytes(int n_cap);
+void sort_scratch_bind(SortScratch &s, int n_cap, void *mem);
+
+// Stable LSD radix sort of keys[0..n) with values (identity slot indices when
+// vals == nullptr).  n is read from device memory.  The sorted result lands in
+// (keys_out, vals_out); ping-pong through (keys_alt, vals_alt) for the
+// intermediate passes.  key_bits selects the number of 8-bit passes.
+void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
+                uint32_t *vals_alt, uint32_t *keys_out, uint32_t *vals_out,
+                const int *n_dev, int key_bits, const SortScratch &s,
+                hipStream_t stream, Profiler *prof);
+
+// Exclusive scan of n ints (n known on the host); partial holds div_up(n, tile).
+void exclusive_scan(const int *in, int *out, int n, int *partial, hipStream_t stream,
+                    Profiler *prof);
+inline int scan_partials(int n) { return div_up(n > 0 ? n : 1, kScanTile) + 1; }
+
+// Stable compaction map: for stencil[i] == 0 (kept), map[j] = i for the j-th
+// kept element; *n_out (device) = number kept.  n read from device memory;
+// n_cap bounds the grid.  partial holds div_up(n_cap, tile) + 1 ints.
+void compaction_map(const int *stencil, const int *n_dev, int n_cap, uint32_t *map,
+                    int *n_out_dev, int *partial, hipStream_t stream, Profiler *prof);
+
+// dst[i] = src[map[i]] for i < *n_dev.
+void gather_u32(uint32_t *dst, const uint32_t *src, const uint32_t *map, const int *n_dev,
+                int n_cap, hipStream_t stream, Profiler *prof);
+
+void fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t stream, Profiler *prof);
+void fill_f64(double *p, double v, long long n, hipStream_t stream, Profiler *prof);
+
+void philox_fill(uint32_t *out, int n, const uint32_t ctr[4], const uint32_t key[2],
+                 hipStream_t stream);
+
+}  // namespace dsmc

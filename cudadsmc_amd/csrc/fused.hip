@@ -1,0 +1,556 @@
+// fused.hip -- FP32 precision form of the DSMC step for any number of
+// independent traps, laid out as one flat work list.
+//
+// Layout.  All traps share one set of structure-of-arrays FP32 buffers; atoms
+// are kept trap-major and, within a trap, in cell order.  A composite cell key
+// key_base[trap] + local_cell (local overflow cell = n_cells of the trap; one
+// global "dead" key after every trap) makes one radix sort order all traps at
+// once and pushes evaporated atoms to the end, so the sort is also the stream
+// compaction and the alive count never has to visit the host.
+//
+// Step (one motion step, lpc = 1):
+//   sort(keys)           -> sorted keys + permutation        (primitives.hip)
+//   k_table_fused        -> cell end per occupied key, trap ranges, alive count
+//   k_collide_fused      -> one thread per occupied cell, state read through
+//                           the permutation (no reorder pass)
+//   k_move_fused         -> gathers state through the permutation into the
+//                           other buffer (so the state ends up sorted), spin
+//                           substeps, Kahan-compensated Verlet, evaporation,
+//                           next cell key, FP64 observables per trap
+// Every kernel reads the alive count and the step number from device memory,
+// so the loop can be captured once and replayed as a HIP graph.
+#include "fused.hpp"
+
+namespace dsmc {
+
+constexpr int kMaxTrapsLds = 8192;
+int fused_max_traps() { return kMaxTrapsLds; }
+
+constexpr int kFBlock = 256;
+
+// Largest t in [0, n) with arr[t] <= v (arr ascending, arr[0] <= v).
+__device__ __forceinline__ int last_le(const int *arr, int n, int v)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (arr[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void load_table(int *lds, const int *src, int cnt)
+{
+    for (int j = threadIdx.x; j < cnt; j += blockDim.x) lds[j] = src[j];
+    __syncthreads();
+}
+
+// s + d with the running remainder c: y = d + c, then an error-free TwoSum of
+// s + y (Knuth), so the remainder stays exact whatever the magnitudes.
+__device__ __forceinline__ void kahan_add(float &s, float &c, float d)
+{
+    const float y = d + c;
+    const float t = s + y;
+    const float bp = t - s;
+    const float ap = t - bp;
+    c = (s - ap) + (y - bp);
+    s = t;
+}
+
+__device__ __forceinline__ void split_df(double v, float &hi, float &lo)
+{
+    hi = (float)v;
+    lo = (float)(v - (double)hi);
+}
+
+// ---------------------------------------------------------------------------
+// Initial distribution (setUp.cu:129-165) into the FP32 state: the FP64 sample
+// is stored as float + float remainder.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kFBlock) void k_init_fused(const TrapDev *__restrict__ traps,
+                                                        const int *__restrict__ trap_start,
+                                                        int T, InitParams ip, FusedPtrs s, int n)
+{
+    extern __shared__ int ts[];
+    load_table(ts, trap_start, T + 1);
+    const int a = blockIdx.x * kFBlock + threadIdx.x;
+    if (a >= n) return;
+    const int t = last_le(ts, T, a);
+    const TrapDev &tr = traps[t];
+    double px, py, pz, vx, vy, vz;
+    init_atom(tr, ip, (uint32_t)(a - ts[t]), px, py, pz, vx, vy, vz);
+    split_df(px, s.p[0][a], s.cp[0][a]);
+    split_df(py, s.p[1][a], s.cp[1][a]);
+    split_df(pz, s.p[2][a], s.cp[2][a]);
+    split_df(vx, s.v[0][a], s.cv[0][a]);
+    split_df(vy, s.v[1][a], s.cv[1][a]);
+    split_df(vz, s.v[2][a], s.cv[2][a]);
+    if (ip.spin) {
+        D3 Bn;
+        (void)field_dir(tr, px, py, pz, Bn);
+        s.s[0][a] = (float)Bn.x; s.s[1][a] = (float)Bn.y; s.s[2][a] = (float)Bn.z;
+    }
+    s.id[a] = a;
+}
+
+// indexAtoms for the FP32 state: the FP64 index of p + cp.
+__global__ __launch_bounds__(kFBlock) void k_index_fused(const TrapDev *__restrict__ traps,
+                                                         const int *__restrict__ trap_start,
+                                                         int T, FusedPtrs s,
+                                                         uint32_t *__restrict__ key,
+                                                         const int *__restrict__ n_dev)
+{
+    extern __shared__ int ts[];
+    load_table(ts, trap_start, T + 1);
+    const int i = blockIdx.x * kFBlock + threadIdx.x;
+    if (i >= *n_dev) return;
+    const int t = last_le(ts, T, i);
+    const TrapDev &tr = traps[t];
+    const double X = (double)s.p[0][i] + (double)s.cp[0][i];
+    const double Y = (double)s.p[1][i] + (double)s.cp[1][i];
+    const double Z = (double)s.p[2][i] + (double)s.cp[2][i];
+    key[i] = (uint32_t)(tr.key_base + cell_of(tr, X, Y, Z));
+}
+
+// ---------------------------------------------------------------------------
+// Cell table from sorted keys: start/end of every occupied key, first slot of
+// every trap, and the alive count (= first dead slot).  n_in + 1 threads: the
+// extra one closes the last trap ranges.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kFBlock) void k_table_fused(const uint32_t *__restrict__ key,
+                                                         const int *__restrict__ key_base,
+                                                         TableParams tp,
+                                                         const int *__restrict__ n_in_dev,
+                                                         int *__restrict__ n_out_dev,
+                                                         uint32_t *__restrict__ cell_start,
+                                                         uint32_t *__restrict__ cell_end,
+                                                         int *__restrict__ trap_start)
+{
+    extern __shared__ int kb[];
+    const int T = tp.n_traps;
+    load_table(kb, key_base, T + 1);
+    const int n = *n_in_dev;
+    const int i = blockIdx.x * kFBlock + threadIdx.x;
+    if (i > n) return;
+    uint32_t k = tp.dead_key;
+    if (i < n) {
+        k = key[i];
+        if (k != tp.dead_key) {
+            if (i == 0 || key[i - 1] != k) cell_start[k] = (uint32_t)i;
+            if (i == n - 1 || key[i + 1] != k) cell_end[k] = (uint32_t)(i + 1);
+        }
+    }
+    const int cur = (i == n) ? T : last_le(kb, T + 1, (int)k);
+    const int prev = (i == 0) ? -1 : last_le(kb, T + 1, (int)key[i - 1]);
+    if (cur != prev) {
+        for (int t = prev + 1; t <= cur; ++t) trap_start[t] = i;
+        if (prev < T && cur == T) *n_out_dev = i;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// collide (collisions.cu:291-445) over occupied cells: the thread of the first
+// slot of each cell runs that cell's trials in sequence.  Trials (pair choice,
+// |v_r|, acceptance) in FP32; an accepted collision updates the pair in FP64
+// from value + remainder and stores the result back as float + remainder, so
+// momentum and energy are conserved to FP64 rounding.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kFBlock) void k_collide_fused(
+    const TrapDev *__restrict__ traps, const int *__restrict__ key_base, int T, CollideParams cp,
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ perm,
+    const int *__restrict__ n_dev, const uint32_t *__restrict__ cell_end, FusedPtrs s,
+    double *__restrict__ sigvrmax, unsigned long long *__restrict__ coll_accum,
+    const long long *__restrict__ step_dev, uint32_t dead_key)
+{
+    extern __shared__ int kb[];
+    load_table(kb, key_base, T + 1);
+    const int i = blockIdx.x * kFBlock + threadIdx.x;
+    if (i >= *n_dev) return;
+    const uint32_t k = key[i];
+    if (k == dead_key || (i > 0 && key[i - 1] == k)) return;
+    const int t = last_le(kb, T + 1, (int)k);
+    const TrapDev &tr = traps[t];
+    const uint32_t cell = k - (uint32_t)tr.key_base;
+    if ((int)cell >= tr.n_cells) return;  // overflow cell: no collisions
+    const int n = (int)cell_end[k] - i;
+    if (n < 2) return;
+    const uint32_t step = (uint32_t)*step_dev;
+    const double lpc = cp.lpc;
+    double smax = sigvrmax[k];
+    const double Mc = 0.5 * (n - 1) * n;
+    const double lambda = ceil(Mc * tr.alpha * lpc * cp.dt * smax / tr.cell_vol) / Mc;
+    const int Ncol = (int)(Mc * lambda);
+    const float pc = (float)(tr.alpha * lpc * cp.dt / tr.cell_vol * cp.sigma / lambda);
+    int count = 0;
+    for (int tr_i = 0; tr_i < Ncol; ++tr_i) {
+        int ia = 0, ib = 1;
+        float u_acc;
+        if (cp.pair_mode == 0) {
+            const U32x4 o = draw(cp.seed, tr.stream, cell, step, (uint32_t)tr_i, kPurposeCollide, 0);
+            ia = (int)uint_below(o.x, (uint32_t)n);
+            ib = (int)uint_below(o.y, (uint32_t)(n - 1));
+            ib += (ib >= ia);
+            u_acc = u01f(o.z);
+        } else {
+            if (n > 2) {
+                ia = 0; ib = 0;
+                for (uint32_t q = 0; q < 64 && ia == ib; ++q) {
+                    const U32x4 o = draw(cp.seed, tr.stream, cell, step, (uint32_t)tr_i, kPurposeCollide, 16 + q);
+                    ia = (int)floor(u01(o.x, o.y) * (n - 1));
+                    ib = (int)floor(u01(o.z, o.w) * (n - 1));
+                }
+                if (ia == ib) { ia = 0; ib = 1; }
+            }
+            const U32x4 o = draw(cp.seed, tr.stream, cell, step, (uint32_t)tr_i, kPurposeCollide, 1);
+            u_acc = u01f(o.x);
+        }
+        const uint32_t a = perm[i + ia], b = perm[i + ib];
+        const float dx = s.v[0][a] - s.v[0][b];
+        const float dy = s.v[1][a] - s.v[1][b];
+        const float dz = s.v[2][a] - s.v[2][b];
+        const float vr = sqrtf(dx * dx + dy * dy + dz * dz);
+        const double sv = (double)vr * cp.sigma;
+        if (sv > smax) smax = sv;
+        if (pc * vr > u_acc) {
+            const double vax = (double)s.v[0][a] + (double)s.cv[0][a];
+            const double vay = (double)s.v[1][a] + (double)s.cv[1][a];
+            const double vaz = (double)s.v[2][a] + (double)s.cv[2][a];
+            const double vbx = (double)s.v[0][b] + (double)s.cv[0][b];
+            const double vby = (double)s.v[1][b] + (double)s.cv[1][b];
+            const double vbz = (double)s.v[2][b] + (double)s.cv[2][b];
+            const double ddx = vax - vbx, ddy = vay - vby, ddz = vaz - vbz;
+            const double magVrel = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+            const double cx = 0.5 * (vax + vbx), cy = 0.5 * (vay + vby), cz = 0.5 * (vaz + vbz);
+            double nx, ny, nz;
+            sphere_point(cp.seed, tr.stream, cell, step, (uint32_t)tr_i, kPurposeCollide, 2, nx, ny, nz);
+            const double wx = magVrel * nx, wy = magVrel * ny, wz = magVrel * nz;
+            split_df(cx - 0.5 * wx, s.v[0][a], s.cv[0][a]);
+            split_df(cy - 0.5 * wy, s.v[1][a], s.cv[1][a]);
+            split_df(cz - 0.5 * wz, s.v[2][a], s.cv[2][a]);
+            split_df(cx + 0.5 * wx, s.v[0][b], s.cv[0][b]);
+            split_df(cy + 0.5 * wy, s.v[1][b], s.cv[1][b]);
+            split_df(cz + 0.5 * wz, s.v[2][b], s.cv[2][b]);
+            ++count;
+        }
+    }
+    sigvrmax[k] = smax;
+    if (count) atomicAdd(&coll_accum[t], (unsigned long long)count);
+}
+
+// ---------------------------------------------------------------------------
+// Fused motion step: gather through the permutation, spin substeps,
+// Kahan-compensated velocity Verlet (moveAtoms.cu:51-91, acceleration
+// recomputed instead of stored), periodic wrap, evaporation (spin projection
+// of evaporation.cu:90-94 and/or energy cut), next cell key, and FP64
+// observables accumulated per trap.  The last block to finish writes one log
+// record per trap and advances the device step counter.
+// ---------------------------------------------------------------------------
+template <int BS>
+__global__ __launch_bounds__(BS) void k_move_fused(
+    const TrapDev *__restrict__ traps, MoveParams mp, FusedPtrs A, FusedPtrs B,
+    const uint32_t *__restrict__ perm, const int *__restrict__ trap_start,
+    const int *__restrict__ n_dev, uint32_t *__restrict__ key_out, float *__restrict__ evap_pos,
+    float *__restrict__ evap_vel, double *__restrict__ acc,
+    unsigned long long *__restrict__ coll_accum, unsigned int *__restrict__ done,
+    long long *__restrict__ step_dev, dsmc_observables *__restrict__ obs_log)
+{
+    extern __shared__ int ts[];
+    __shared__ double red[(BS / kWave) * kAccK];
+    __shared__ bool last;
+    const int T = mp.n_traps;
+    load_table(ts, trap_start, T + 1);
+    const int n = *n_dev;
+    const long long step = *step_dev;
+    const int i = blockIdx.x * BS + threadIdx.x;
+    double vals[kAccK];
+#pragma unroll
+    for (int q = 0; q < kAccK; ++q) vals[q] = 0.0;
+    int t = -1;
+    if (i < n) {
+        t = last_le(ts, T, i);
+        const TrapDev &tr = traps[t];
+        const uint32_t j = perm ? perm[i] : (uint32_t)i;
+        float x = A.p[0][j], y = A.p[1][j], z = A.p[2][j];
+        float cx = A.cp[0][j], cy = A.cp[1][j], cz = A.cp[2][j];
+        float vx = A.v[0][j], vy = A.v[1][j], vz = A.v[2][j];
+        float cvx = A.cv[0][j], cvy = A.cv[1][j], cvz = A.cv[2][j];
+        const int id = A.id[j];
+        const float dt = mp.f_dt;
+        const float hdt = 0.5f * dt;
+
+        F3 a0 = accel_f(tr, x, y, z);
+        kahan_add(vx, cvx, a0.x * hdt);
+        kahan_add(vy, cvy, a0.y * hdt);
+        kahan_add(vz, cvz, a0.z * hdt);
+
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        if (mp.spin) {
+            sx = A.s[0][j]; sy = A.s[1][j]; sz = A.s[2][j];
+            const double dts = mp.dt / mp.substeps;
+            const float w = (float)(mp.gs * kMuB * dts / kHbar);
+            const float fdts = (float)dts;
+            for (int m = 0; m < mp.substeps; ++m) {
+                const float tau = mp.interp ? ((float)m + 0.5f) * fdts : 0.0f;
+                F3 Bn;
+                const float Bm = field_dir_f(tr, x + vx * tau, y + vy * tau, z + vz * tau, Bn);
+                float sn, cs;
+                sincosf(w * Bm, &sn, &cs);
+                const float nd = Bn.x * sx + Bn.y * sy + Bn.z * sz;
+                const float kx = Bn.y * sz - Bn.z * sy;
+                const float ky = Bn.z * sx - Bn.x * sz;
+                const float kz = Bn.x * sy - Bn.y * sx;
+                const float om = nd * (1.0f - cs);
+                sx = sx * cs + kx * sn + Bn.x * om;
+                sy = sy * cs + ky * sn + Bn.y * om;
+                sz = sz * cs + kz * sn + Bn.z * om;
+            }
+            const double L = sqrt((double)sx * sx + (double)sy * sy + (double)sz * sz);
+            sx = (float)((double)sx / L); sy = (float)((double)sy / L); sz = (float)((double)sz / L);
+        }
+
+        kahan_add(x, cx, vx * dt);
+        kahan_add(y, cy, vy * dt);
+        kahan_add(z, cz, vz * dt);
+        if (tr.periodic) {  // wrap the FP64 value, then split it again
+            double X = (double)x + (double)cx, Y = (double)y + (double)cy, Z = (double)z + (double)cz;
+            if (X >= tr.hw[0] || X < -tr.hw[0]) { X = wrap_periodic(X, tr.hw[0]); split_df(X, x, cx); }
+            if (Y >= tr.hw[1] || Y < -tr.hw[1]) { Y = wrap_periodic(Y, tr.hw[1]); split_df(Y, y, cy); }
+            if (Z >= tr.hw[2] || Z < -tr.hw[2]) { Z = wrap_periodic(Z, tr.hw[2]); split_df(Z, z, cz); }
+        }
+        F3 a1 = accel_f(tr, x, y, z);
+        kahan_add(vx, cvx, a1.x * hdt);
+        kahan_add(vy, cvy, a1.y * hdt);
+        kahan_add(vz, cvz, a1.z * hdt);
+
+        const double X = (double)x + (double)cx, Y = (double)y + (double)cy, Z = (double)z + (double)cz;
+        const double VX = (double)vx + (double)cvx, VY = (double)vy + (double)cvy,
+                     VZ = (double)vz + (double)cvz;
+        const double ek = 0.5 * mp.mass * (VX * VX + VY * VY + VZ * VZ);
+        const double ep = potential_energy(tr, X, Y, Z);
+        bool lost = false;
+        if (mp.spin_loss) {
+            F3 Bn;
+            (void)field_dir_f(tr, x, y, z, Bn);
+            lost = Bn.x * sx + Bn.y * sy + Bn.z * sz < 0.0f;
+        }
+        if (mp.energy_cut && tr.kT_cut > 0.0) {
+            const double f = tr.eta_steps > 0 ? fmin((double)step / (double)tr.eta_steps, 1.0) : 1.0;
+            const double eta = tr.eta_start + (tr.eta_end - tr.eta_start) * f;
+            lost = lost || (ek + ep > eta * tr.kT_cut);
+        }
+        uint32_t kout;
+        if (lost) {
+            kout = mp.dead_key;
+            evap_pos[3 * (size_t)id + 0] = (float)X; evap_pos[3 * (size_t)id + 1] = (float)Y;
+            evap_pos[3 * (size_t)id + 2] = (float)Z;
+            evap_vel[3 * (size_t)id + 0] = (float)VX; evap_vel[3 * (size_t)id + 1] = (float)VY;
+            evap_vel[3 * (size_t)id + 2] = (float)VZ;
+            vals[kAccEevap] = ek + ep;
+            vals[kAccNevap] = 1.0;
+        } else {
+            kout = (uint32_t)(tr.key_base + cell_of(tr, X, Y, Z));
+            vals[kAccEk] = ek;
+            vals[kAccEp] = ep;
+            vals[kAccV2x] = VX * VX; vals[kAccV2y] = VY * VY; vals[kAccV2z] = VZ * VZ;
+            vals[kAccN] = 1.0;
+        }
+        B.p[0][i] = x; B.p[1][i] = y; B.p[2][i] = z;
+        B.cp[0][i] = cx; B.cp[1][i] = cy; B.cp[2][i] = cz;
+        B.v[0][i] = vx; B.v[1][i] = vy; B.v[2][i] = vz;
+        B.cv[0][i] = cvx; B.cv[1][i] = cvy; B.cv[2][i] = cvz;
+        if (mp.spin) { B.s[0][i] = sx; B.s[1][i] = sy; B.s[2][i] = sz; }
+        B.id[i] = id;
+        key_out[i] = kout;
+    }
+
+    // Per-trap block sums (atoms are trap-major, so a block spans a few traps).
+    const int b0 = blockIdx.x * BS;
+    if (b0 < n) {
+        const int t_first = last_le(ts, T, b0);
+        const int t_last = last_le(ts, T, min(n, b0 + BS) - 1);
+        for (int tt = t_first; tt <= t_last; ++tt) {
+            double v[kAccK];
+#pragma unroll
+            for (int q = 0; q < kAccK; ++q) v[q] = (t == tt) ? vals[q] : 0.0;
+            block_sum<BS, kAccK>(v, red);
+            if (threadIdx.x == 0) {
+#pragma unroll
+                for (int q = 0; q < kAccK; ++q)
+                    if (v[q] != 0.0) atomicAdd(&acc[tt * kAccK + q], v[q]);
+            }
+        }
+    }
+
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const long long slot = step % mp.obs_cap;
+    for (int tt = threadIdx.x; tt < T; tt += BS) {
+        double a[kAccK];
+#pragma unroll
+        for (int q = 0; q < kAccK; ++q) {
+            a[q] = __hip_atomic_load(&acc[tt * kAccK + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&acc[tt * kAccK + q], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        const unsigned long long c =
+            __hip_atomic_load(&coll_accum[tt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&coll_accum[tt], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        dsmc_observables &o = obs_log[slot * T + tt];
+        o.ek = a[kAccEk];
+        o.ep = a[kAccEp];
+        o.v2[0] = a[kAccV2x]; o.v2[1] = a[kAccV2y]; o.v2[2] = a[kAccV2z];
+        o.e_evap = a[kAccEevap];
+        o.n_alive = (long long)a[kAccN] + (long long)a[kAccNevap];
+        o.n_evap = (long long)a[kAccNevap];
+        o.collisions = (long long)c;
+        o.step = step;
+    }
+    if (threadIdx.x == 0) {
+        *done = 0u;
+        *step_dev = step + 1;
+    }
+}
+
+// dst[i] = src[map[i]] for every FP32 state array.
+__global__ __launch_bounds__(kFBlock) void k_gather_fused(FusedPtrs dst, FusedPtrs src,
+                                                          const uint32_t *__restrict__ map,
+                                                          const int *__restrict__ n_dev, int spin)
+{
+    const int i = blockIdx.x * kFBlock + threadIdx.x;
+    if (i >= *n_dev) return;
+    const uint32_t j = map[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        dst.p[c][i] = src.p[c][j]; dst.v[c][i] = src.v[c][j];
+        dst.cp[c][i] = src.cp[c][j]; dst.cv[c][i] = src.cv[c][j];
+        if (spin) dst.s[c][i] = src.s[c][j];
+    }
+    dst.id[i] = src.id[j];
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+static int fgrid(long long n, int bs) { return div_up(n > 0 ? n : 1, bs); }
+
+void launch_init_fused(const TrapDev *traps, const int *trap_start, int n_traps,
+                       const InitParams &ip, const FusedPtrs &s, int n, hipStream_t st,
+                       Profiler *prof)
+{
+    const size_t lds = sizeof(int) * (size_t)(n_traps + 1);
+    Profiler *p_ = prof;
+    if (p_ && p_->enabled) p_->begin(st, "init_fused", (double)n * 60.0);
+    hipLaunchKernelGGL(k_init_fused, dim3(fgrid(n, kFBlock)), dim3(kFBlock), lds, st, traps,
+                       trap_start, n_traps, ip, s, n);
+    DSMC_HIP(hipGetLastError());
+    if (p_ && p_->enabled) p_->end(st);
+}
+
+void launch_index_fused(const TrapDev *traps, const int *trap_start, int n_traps,
+                        const FusedPtrs &s, uint32_t *key, const int *n_dev, int n_cap,
+                        hipStream_t st, Profiler *prof)
+{
+    const size_t lds = sizeof(int) * (size_t)(n_traps + 1);
+    if (prof && prof->enabled) prof->begin(st, "index_fused", (double)n_cap * 28.0);
+    hipLaunchKernelGGL(k_index_fused, dim3(fgrid(n_cap, kFBlock)), dim3(kFBlock), lds, st, traps,
+                       trap_start, n_traps, s, key, n_dev);
+    DSMC_HIP(hipGetLastError());
+    if (prof && prof->enabled) prof->end(st);
+}
+
+void launch_table_fused(const uint32_t *key, const int *key_base, const TableParams &tp,
+                        const int *n_in, int *n_out, uint32_t *cell_start, uint32_t *cell_end,
+                        int *trap_start, int n_cap, hipStream_t st, Profiler *prof)
+{
+    const size_t lds = sizeof(int) * (size_t)(tp.n_traps + 1);
+    if (prof && prof->enabled) prof->begin(st, "table_fused", (double)n_cap * 4.0);
+    hipLaunchKernelGGL(k_table_fused, dim3(fgrid((long long)n_cap + 1, kFBlock)), dim3(kFBlock),
+                       lds, st, key, key_base, tp, n_in, n_out, cell_start, cell_end, trap_start);
+    DSMC_HIP(hipGetLastError());
+    if (prof && prof->enabled) prof->end(st);
+}
+
+void launch_collide_fused(const TrapDev *traps, const int *key_base, int n_traps,
+                          const CollideParams &cp, const uint32_t *key, const uint32_t *perm,
+                          const int *n_dev, const uint32_t *cell_end, const FusedPtrs &s,
+                          double *sigvrmax, unsigned long long *coll_accum,
+                          const long long *step_dev, uint32_t dead_key, int n_cap,
+                          hipStream_t st, Profiler *prof)
+{
+    const size_t lds = sizeof(int) * (size_t)(n_traps + 1);
+    if (prof && prof->enabled) prof->begin(st, "collide_fused", (double)n_cap * 8.0);
+    hipLaunchKernelGGL(k_collide_fused, dim3(fgrid(n_cap, kFBlock)), dim3(kFBlock), lds, st,
+                       traps, key_base, n_traps, cp, key, perm, n_dev, cell_end, s, sigvrmax,
+                       coll_accum, step_dev, dead_key);
+    DSMC_HIP(hipGetLastError());
+    if (prof && prof->enabled) prof->end(st);
+}
+
+double move_fused_bytes_per_atom(int spin)
+{
+    // read: perm 4 + 12 floats + id 4 (+ 3 spin floats); write: same + key 4
+    const double state = 12 * 4 + 4 + (spin ? 12 : 0);
+    return 4.0 + state + state + 4.0;
+}
+
+template <int BS>
+static void move_fused_bs(const TrapDev *traps, const MoveParams &mp, const FusedPtrs &src,
+                          const FusedPtrs &dst, const uint32_t *perm, const int *trap_start,
+                          const int *n_dev, uint32_t *key_out, float *evap_pos, float *evap_vel,
+                          double *acc, unsigned long long *coll_accum, unsigned int *done,
+                          long long *step_dev, dsmc_observables *obs_log, int n_cap,
+                          hipStream_t st)
+{
+    const size_t lds = sizeof(int) * (size_t)(mp.n_traps + 1);
+    hipLaunchKernelGGL(k_move_fused<BS>, dim3(fgrid(n_cap, BS)), dim3(BS), lds, st, traps, mp, src,
+                       dst, perm, trap_start, n_dev, key_out, evap_pos, evap_vel, acc, coll_accum,
+                       done, step_dev, obs_log);
+}
+
+void launch_move_fused(int block, const TrapDev *traps, const MoveParams &mp, const FusedPtrs &src,
+                       const FusedPtrs &dst, const uint32_t *perm, const int *trap_start,
+                       const int *n_dev, uint32_t *key_out, float *evap_pos, float *evap_vel,
+                       double *acc, unsigned long long *coll_accum, unsigned int *done,
+                       long long *step_dev, dsmc_observables *obs_log, int n_cap,
+                       hipStream_t st, Profiler *prof)
+{
+    if (prof && prof->enabled)
+        prof->begin(st, "move_fused", (double)n_cap * move_fused_bytes_per_atom(mp.spin));
+    switch (block) {
+    case 64:
+        move_fused_bs<64>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos, evap_vel,
+                          acc, coll_accum, done, step_dev, obs_log, n_cap, st);
+        break;
+    case 128:
+        move_fused_bs<128>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
+                           evap_vel, acc, coll_accum, done, step_dev, obs_log, n_cap, st);
+        break;
+    case 512:
+        move_fused_bs<512>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
+                           evap_vel, acc, coll_accum, done, step_dev, obs_log, n_cap, st);
+        break;
+    case 1024:
+        move_fused_bs<1024>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
+                            evap_vel, acc, coll_accum, done, step_dev, obs_log, n_cap, st);
+        break;
+    default:
+        move_fused_bs<256>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
+                           evap_vel, acc, coll_accum, done, step_dev, obs_log, n_cap, st);
+        break;
+    }
+    DSMC_HIP(hipGetLastError());
+    if (prof && prof->enabled) prof->end(st);
+}
+
+void launch_gather_fused(const FusedPtrs &dst, const FusedPtrs &src, const uint32_t *map,
+                         const int *n_dev, int n_cap, int spin, hipStream_t st, Profiler *prof)
+{
+    DSMC_LAUNCH(prof, "gather_fused", (double)n_cap * (4.0 + 2.0 * (52 + (spin ? 12 : 0))),
+                k_gather_fused, fgrid(n_cap, kFBlock), kFBlock, st, dst, src, map, n_dev, spin);
+}
+
+}  // namespace dsmc

@@ -1,0 +1,74 @@
+// fused.hpp -- kernel parameter structs and host launchers of the FP32
+// precision form (Kahan-compensated state, any number of traps).
+#pragma once
+
+#include "../../include/dsmc.h"
+#include "dsmc_device.hpp"
+#include "dsmc_internal.hpp"
+#include "init_device.hpp"
+#include "plain.hpp"
+
+namespace dsmc {
+
+// Device pointers of one FP32 state buffer (structure of arrays).  The value
+// of each coordinate is p + cp (and v + cv): the float `p` plus the Kahan
+// remainder `cp` that the previous updates could not represent.
+struct FusedPtrs {
+    float *p[3];
+    float *v[3];
+    float *cp[3];
+    float *cv[3];
+    float *s[3];     // Bloch spin vector (spin runs only)
+    int   *id;       // atom label of each slot
+};
+
+// Observable accumulators per trap (doubles).
+enum { kAccEk = 0, kAccEp, kAccV2x, kAccV2y, kAccV2z, kAccEevap, kAccNevap, kAccN, kAccK };
+
+struct MoveParams {
+    double dt;
+    double mass;
+    double gs;
+    float  f_dt;
+    int    spin;
+    int    substeps;
+    int    interp;
+    int    spin_loss;
+    int    energy_cut;
+    int    n_traps;
+    int    obs_cap;      // records in the observables ring (per trap)
+    uint32_t dead_key;
+};
+
+struct TableParams {
+    int n_traps;
+    uint32_t dead_key;
+};
+
+int  fused_max_traps();
+void launch_init_fused(const TrapDev *traps, const int *trap_start, int n_traps,
+                       const InitParams &ip, const FusedPtrs &s, int n, hipStream_t st,
+                       Profiler *prof);
+void launch_index_fused(const TrapDev *traps, const int *trap_start, int n_traps,
+                        const FusedPtrs &s, uint32_t *key, const int *n_dev, int n_cap,
+                        hipStream_t st, Profiler *prof);
+void launch_table_fused(const uint32_t *key, const int *key_base, const TableParams &tp,
+                        const int *n_in, int *n_out, uint32_t *cell_start, uint32_t *cell_end,
+                        int *trap_start, int n_cap, hipStream_t st, Profiler *prof);
+void launch_collide_fused(const TrapDev *traps, const int *key_base, int n_traps,
+                          const CollideParams &cp, const uint32_t *key, const uint32_t *perm,
+                          const int *n_dev, const uint32_t *cell_end, const FusedPtrs &s,
+                          double *sigvrmax, unsigned long long *coll_accum,
+                          const long long *step_dev, uint32_t dead_key, int n_cap,
+                          hipStream_t st, Profiler *prof);
+void launch_move_fused(int block, const TrapDev *traps, const MoveParams &mp, const FusedPtrs &src,
+                       const FusedPtrs &dst, const uint32_t *perm, const int *trap_start,
+                       const int *n_dev, uint32_t *key_out, float *evap_pos, float *evap_vel,
+                       double *acc, unsigned long long *coll_accum, unsigned int *done,
+                       long long *step_dev, dsmc_observables *obs_log, int n_cap,
+                       hipStream_t st, Profiler *prof);
+void launch_gather_fused(const FusedPtrs &dst, const FusedPtrs &src, const uint32_t *map,
+                         const int *n_dev, int n_cap, int spin, hipStream_t st, Profiler *prof);
+double move_fused_bytes_per_atom(int spin);
+
+}  // namespace dsmc

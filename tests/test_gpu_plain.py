@@ -1,0 +1,152 @@
+"""GPU parity of the FP64 drop-in form against the CPU oracle.
+
+Both sides start from the same positions and velocities (sampled by the
+oracle), use the same Philox counters and the same FP64 operand order with no
+fused multiply-adds, so cell ids, the sort permutation, the cell table, the
+compaction survivors and (on the quadrupole and box traps) the whole FP64
+trajectory including collisions are compared bit for bit.  Spin rotations use
+sin/cos, whose last-ulp results differ between the CPU libm and the device
+library, so spinors are compared to 1e-13.
+"""
+import numpy as np
+import pytest
+
+from cudadsmc_amd import Simulation, _abi, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def quad_run(n=20000, nc=35, **kw):
+    run = configs.Run(traps=[configs.quadrupole_trap(n, nc)], precision="fp64", dt=1e-6, seed=7,
+                      name="test-quad")
+    return run.with_(**kw) if kw else run
+
+
+def start(orc, run):
+    """Oracle-sampled initial state loaded into a fresh GPU context."""
+    ref = orc.Reference(run)
+    sim = Simulation(run)
+    sim.set_state(ref.s["pos"], ref.s["vel"])
+    return ref, sim
+
+
+def by_label(sim, field):
+    """GPU per-slot field reordered to atom-label order (the reference's
+    physical-index layout)."""
+    ids = sim.copy_out(_abi.FIELD_ATOM_ID)
+    a = sim.copy_out(field)
+    out = np.empty_like(a)
+    out[ids] = a
+    return out, ids
+
+
+def test_index_sort_table_bit_exact(orc):
+    run = quad_run()
+    print("index/sort/table quadrupole N=20000 35^3", flush=True)
+    ref, sim = start(orc, run)
+    with sim:
+        sim.index_atoms()
+        cell_gpu = sim.copy_out(_abi.FIELD_CELL_ID)
+        cell_orc = orc.index_atoms(ref.p, ref.s["pos"], ref.s["atom_id"])
+        np.testing.assert_array_equal(cell_gpu, cell_orc)
+
+        sim.sort_arrays()
+        k_orc, id_orc = orc.sort_arrays(cell_orc, ref.s["atom_id"], orc.ncells(ref.p) + 1)
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_CELL_ID), k_orc)
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_ATOM_ID), id_orc)
+
+        sim.build_cell_table()
+        se, count, prefix = orc.cell_table(k_orc, orc.ncells(ref.p))
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_CELL_START_END), se)
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_CELL_COUNT), count)
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_CELL_PREFIX), prefix)
+
+
+@pytest.mark.parametrize("pair_mode", ["uniform", "reference"])
+def test_collide_bit_exact(orc, pair_mode):
+    # dense cells so that every cell runs many trials and accepts often
+    run = quad_run(n=50000, nc=12, pair_mode=pair_mode)
+    run.traps[0].n_real = 5e9
+    print(f"collide bit-exact {pair_mode}", flush=True)
+    ref, sim = start(orc, run)
+    with sim:
+        sim.index_atoms(); sim.sort_arrays(); sim.build_cell_table(); sim.collide()
+        info = ref.collision_step()
+        assert info["total"] > 1000
+        vel_gpu, ids = by_label(sim, _abi.FIELD_VEL)
+        np.testing.assert_array_equal(vel_gpu, ref.s["vel"])
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_COLLISION_COUNT)[:-1],
+                                      info["coll_count"][:-1])
+        sv = sim.copy_out(_abi.FIELD_SIGVRMAX)
+        np.testing.assert_array_equal(sv[: len(ref.sigvrmax)], ref.sigvrmax)
+
+
+@pytest.mark.parametrize("kind", ["quadrupole", "ip", "box"])
+def test_trajectory_bit_exact(orc, kind):
+    if kind == "quadrupole":
+        run = quad_run(n=20000, nc=30)
+    elif kind == "ip":
+        run = configs.reference(20, 20000).with_(spin=False, spin_loss=False, lpc=1, dt=1e-6,
+                                                 pair_mode="uniform")
+    else:
+        run = configs.box_thermalise("fp64", n_atoms=30000, nc=24)
+    print(f"trajectory bit-exact {kind}", flush=True)
+    ref, sim = start(orc, run)
+    with sim:
+        steps = 40
+        sim.step(steps)
+        ref.step(steps)
+        pos, ids = by_label(sim, _abi.FIELD_POS)
+        vel, _ = by_label(sim, _abi.FIELD_VEL)
+        acc, _ = by_label(sim, _abi.FIELD_ACC)
+        np.testing.assert_array_equal(np.sort(ids), np.arange(run.n_atoms))
+        np.testing.assert_array_equal(pos, ref.s["pos"])
+        np.testing.assert_array_equal(vel, ref.s["vel"])
+        np.testing.assert_array_equal(acc, ref.s["acc"])
+        log = sim.obs_log()
+        assert int(log["collisions"].sum()) == ref.collisions
+        e = orc.energy(ref.p, ref.s["pos"], ref.s["vel"], ref.s["atom_id"])
+        np.testing.assert_allclose(log["ek"][-1, 0], e[0], rtol=1e-12)
+        np.testing.assert_allclose(log["ep"][-1, 0], e[1], rtol=1e-12)
+
+
+def test_reference_run_spin_and_evaporation(orc):
+    """The reference's own loop (IP trap, spinors, spin-projection loss, lpc 10,
+    biased pair choice) against the oracle for 30 motion steps."""
+    run = configs.reference(20, 20000)
+    print("reference loop with spin + evaporation", flush=True)
+    ref = orc.Reference(run)
+    # tilt a tenth of the spinors so that some atoms are lost
+    psi_u, psi_d = ref.s["psi_up"], ref.s["psi_dn"]
+    flip = np.arange(run.n_atoms) % 10 == 3
+    psi_u[flip], psi_d[flip] = psi_d[flip].copy(), -psi_u[flip].copy()
+    sim = Simulation(run)
+    with sim:
+        sim.set_state(ref.s["pos"], ref.s["vel"])
+        sim.set_spin(np.concatenate([psi_u, psi_d], axis=1))
+        sim.step(30)
+        ref.step(30)
+        ids = sim.copy_out(_abi.FIELD_ATOM_ID)
+        assert set(ids.tolist()) == set(ref.s["atom_id"].tolist())
+        assert len(ids) < run.n_atoms
+        pos, _ = by_label(sim, _abi.FIELD_POS)
+        alive = ref.s["atom_id"]
+        np.testing.assert_allclose(pos[alive], ref.s["pos"][alive], rtol=0, atol=1e-18)
+        spin = sim.copy_out(_abi.FIELD_SPIN)
+        sp = np.empty_like(spin)
+        sp[ids] = spin
+        np.testing.assert_allclose(sp[alive, :2], psi_u[alive], rtol=0, atol=1e-13)
+        np.testing.assert_allclose(sp[alive, 2:], psi_d[alive], rtol=0, atol=1e-13)
+
+
+def test_init_distribution_matches_oracle(orc):
+    run = quad_run(n=20000)
+    print("init distribution", flush=True)
+    with Simulation(run) as sim:
+        sim.init_distribution()
+        pos, ids = by_label(sim, _abi.FIELD_POS)
+        vel, _ = by_label(sim, _abi.FIELD_VEL)
+    p = orc.params_from(run)
+    st = orc.init_distribution(p, run.n_atoms, run.traps[0].T)
+    np.testing.assert_allclose(pos, st["pos"], rtol=1e-13, atol=1e-20)
+    np.testing.assert_allclose(vel, st["vel"], rtol=1e-13, atol=1e-18)
