@@ -1,0 +1,220 @@
+"""Benchmark of the DSMC step on one MI355X; prints ONE JSON line.
+
+Headline (`value`): atom-steps/s of the FP32 Kahan form on the batched
+workload -- 256 independent quadrupole traps, N log-uniform in [1e3, 1e5],
+B' ~ U[1, 5] T/m, T ~ U[5, 50] uK, per-trap 32^3..64^3 cell grids and energy
+cuts, collisions on, dt = 1e-6 s -- replayed through a HIP graph.
+atom-steps/s = (sum over timed steps and traps of atoms moved) / (hipEvent
+time of exactly --steps steps on the context's stream), state resident in HBM.
+
+Also reported: the FP64 plain (drop-in) form and the FP32 form on the single
+quadrupole trap N = 1e6 / 100^3 cells (the precision-form workload), their
+ratio, the FP64 energy drift of the FP32 run, a per-kernel roofline from
+hipEvents around every launch of a separate profiled run, and the CPU oracle
+(OpenMP FP64 restatement of the reference kernels) on N = 1e4 / 35^3.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per atom (or per pass) of each kernel of the FP32 form.
+#   move_fused: read perm (4) + 12 floats of state (48) + label (4);
+#               write 12 floats (48) + label (4) + next cell key (4)
+#   radix_downsweep: read key + value, write key + value (first pass reads no
+#               value: identity); radix_upsweep: read key
+#   table_fused: read key; collide_fused: read key (per slot) -- per-cell work
+#               is counted by the measured traffic, not here.
+PER_ATOM_BYTES = {
+    "move_fused": 112.0,
+    "radix_downsweep": 16.0,
+    "radix_upsweep": 4.0,
+    "table_fused": 4.0,
+    "collide_fused": 4.0,
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def atom_steps(sim, steps):
+    lg = sim.obs_log()[-steps:]
+    return int(lg["n_alive"].sum())
+
+
+def run_gpu_leg(run, steps, warmup, autotune=True):
+    from cudadsmc_amd import Simulation
+
+    sim = Simulation(run)
+    sim.init_distribution()
+    bs = sim.autotune() if (autotune and run.precision == "fp32") else 256
+    sim.step(warmup)
+    sim.sync()
+    ms = sim.time_steps(steps)
+    n = atom_steps(sim, steps)
+    return sim, dict(ms=ms, atom_steps=n, value=n / (ms * 1e-3), block=bs)
+
+
+def energy_drift(sim, steps):
+    lg = sim.obs_log()[-steps:, 0]
+    E = lg["ek"] + lg["ep"] + np.cumsum(lg["e_evap"])
+    return float(np.max(np.abs(E - E[0])) / abs(E[0]))
+
+
+def profile_leg(sim, steps):
+    """Per-kernel hipEvent timing over `steps` eager steps."""
+    sim.profile(True)
+    sim.step(steps)
+    stats = sim.profile_read()
+    sim.profile(False)
+    lg = sim.obs_log()[-steps:]
+    n_mean = float(lg["n_alive"].sum(axis=1).mean())
+    out = {}
+    for name, st in stats.items():
+        avg_ms = st["total_ms"] / max(st["launches"], 1)
+        rec = dict(launches=st["launches"], total_ms=round(st["total_ms"], 4),
+                   avg_us=round(avg_ms * 1e3, 3))
+        if name in PER_ATOM_BYTES:
+            b = PER_ATOM_BYTES[name] * n_mean
+            if name == "radix_downsweep":
+                # first pass of each sort reads no values
+                passes = sim.info().radix_passes
+                b = n_mean * (16.0 * passes - 4.0) / passes
+            rec["alg_bytes"] = b
+            rec["achieved_GBs"] = round(b / (avg_ms * 1e-3) / 1e9, 1)
+        out[name] = rec
+    return out, n_mean
+
+
+def cpu_baseline(target_s=12.0):
+    from cudadsmc_amd import configs
+    from oracle import oracle as orc
+
+    run = configs.quadrupole_1e4()
+    p = orc.params_from(run)
+    t = run.traps[0]
+    st = orc.init_distribution(p, t.n_atoms, t.T)
+    sv = np.full(orc.ncells(p) + 1, orc.sigvrmax_init(p, t.T))
+    probe = orc.run(p, st, 50, 0, sv)
+    per_step = probe["seconds"] / 50
+    n = int(max(50, min(5000, target_s / max(per_step, 1e-6))))
+    res = orc.run(p, st, n, 50, sv)
+    return dict(value=res["atom_steps"] / res["seconds"], unit="atom-steps/s",
+                cores=res["threads"], kind="port",
+                sample=f"N=1e4 quadrupole, 35^3 cells, dt=1e-6 s, collisions on, FP64, "
+                       f"{n} steps ({res['seconds']:.1f} s) of the OpenMP oracle")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--single-steps", type=int, default=2000)
+    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="batched,fp64,fp32")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs one GPU: the DSMC step is measured on a single MI355X; "
+              "multi-GPU sharding is a design note only (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from cudadsmc_amd import configs, load_library
+
+    load_library()
+    legs = set(args.legs.split(","))
+    result = {}
+
+    batched = configs.batched_traps(steps=args.steps)
+    batched.obs_capacity = args.steps + args.warmup + args.profile_steps + 64
+    roof = None
+    kernels = None
+    if "batched" in legs:
+        log(f"[bench] batched: {len(batched.traps)} traps, {batched.n_atoms} atoms")
+        sim, b = run_gpu_leg(batched, args.steps, args.warmup)
+        log(f"[bench] batched FP32: {b['value']:.4e} atom-steps/s ({b['ms']:.1f} ms, block {b['block']})")
+        result["batched"] = b
+        kernels, n_mean = profile_leg(sim, args.profile_steps)
+        info = sim.info()
+        sim.close()
+        dom = max((k for k in kernels if "alg_bytes" in kernels[k]),
+                  key=lambda k: kernels[k]["total_ms"])
+        d = kernels[dom]
+        achieved = d["alg_bytes"] / (d["avg_us"] * 1e-6) / 1e9
+        roof = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    alg_bytes_per_launch=round(d["alg_bytes"]), avg_launch_us=d["avg_us"],
+                    radix_passes=info.radix_passes)
+        log(f"[bench] dominant kernel {dom}: {achieved:.0f} GB/s")
+    if "fp64" in legs:
+        run64 = configs.quadrupole_1e6("fp64").with_(obs_capacity=args.single_steps + args.warmup + 64)
+        sim, r = run_gpu_leg(run64, args.single_steps, args.warmup)
+        sim.close()
+        log(f"[bench] FP64 plain N=1e6: {r['value']:.4e} atom-steps/s")
+        result["fp64_plain"] = r
+    if "fp32" in legs:
+        run32 = configs.quadrupole_1e6("fp32").with_(obs_capacity=args.single_steps + args.warmup + 64)
+        sim, r = run_gpu_leg(run32, args.single_steps, args.warmup)
+        r["energy_drift"] = energy_drift(sim, args.single_steps)
+        sim.close()
+        log(f"[bench] FP32 Kahan N=1e6: {r['value']:.4e} atom-steps/s, |dE/E| {r['energy_drift']:.2e}")
+        result["fp32_single"] = r
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    if cpu:
+        log(f"[bench] CPU oracle: {cpu['value']:.4e} atom-steps/s on {cpu['cores']} threads")
+
+    head = result.get("batched") or result.get("fp32_single") or result.get("fp64_plain")
+    steps = args.steps if "batched" in result else args.single_steps
+    line = {
+        "metric": "atom-steps/s (collisions on), FP32-batched vs FP64-plain at |dE/E|<=1e-4",
+        "value": round(head["value"], 1),
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms"] / steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Philox-sampled thermal clouds)",
+        "config": {
+            "workload": "batched_traps" if "batched" in result else "quadrupole_1e6",
+            "n_traps": len(batched.traps) if "batched" in result else 1,
+            "n_atoms": batched.n_atoms if "batched" in result else 1000000,
+            "dt": 1e-6,
+            "precision": "fp32-kahan, fp64 reductions",
+            "parallelism": "single-gpu",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if "fp64_plain" in result:
+        line["fp64_plain"] = {k: round(v, 3) if isinstance(v, float) else v
+                              for k, v in result["fp64_plain"].items()}
+        line["ratio_vs_fp64_plain"] = round(head["value"] / result["fp64_plain"]["value"], 3)
+    if "fp32_single" in result:
+        line["fp32_single"] = {k: (round(v, 3) if isinstance(v, float) and k != "energy_drift" else v)
+                               for k, v in result["fp32_single"].items()}
+    if kernels:
+        line["kernels"] = kernels
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -45,7 +45,8 @@ class Config(C.Structure):
         ("spin", C.c_int), ("spin_substeps", C.c_int), ("spin_interp", C.c_int),
         ("lpc", C.c_int), ("use_graph", C.c_int),
         ("dt", C.c_double), ("mass", C.c_double), ("sigma", C.c_double), ("gs", C.c_double),
-        ("seed", C.c_ulonglong), ("block_size", C.c_int), ("reserved", C.c_int * 7),
+        ("seed", C.c_ulonglong), ("block_size", C.c_int), ("obs_capacity", C.c_int),
+        ("reserved", C.c_int * 6),
     ]
 
 

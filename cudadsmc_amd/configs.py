@@ -103,6 +103,7 @@ class Run:
     seed: int = 1
     use_graph: bool = True
     block_size: int = 0
+    obs_capacity: int = 0
     steps: int = 1000
     name: str = "custom"
 
@@ -130,6 +131,7 @@ class Run:
         c.dt, c.mass, c.sigma, c.gs = self.dt, self.mass, self.sigma, self.gs
         c.seed = int(self.seed)
         c.block_size = int(self.block_size)
+        c.obs_capacity = int(self.obs_capacity)
         return c
 
 

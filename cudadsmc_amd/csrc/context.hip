@@ -241,6 +241,8 @@ static void validate(const dsmc_config *c, const dsmc_trap *traps, int n_traps)
     if (!(c->dt > 0.0) || !(c->mass > 0.0) || !(c->sigma >= 0.0))
         fail(DSMC_EINVAL, "dt and mass must be > 0, sigma >= 0");
     if (c->lpc < 1) fail(DSMC_EINVAL, "lpc must be >= 1");
+    if (c->obs_capacity < 0 || (long long)c->obs_capacity * n_traps > (1LL << 28))
+        fail(DSMC_EINVAL, "obs_capacity out of range");
     if (c->precision == DSMC_FP32_KAHAN && c->lpc != 1)
         fail(DSMC_ENOTSUP, "the FP32 form collides every motion step (lpc = 1)");
     if (c->spin && c->spin_substeps < 1) fail(DSMC_EINVAL, "spin_substeps must be >= 1");
@@ -306,7 +308,9 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
     x->key_bits = bits;
     x->passes = bits <= kRadixBits ? 1 : div_up(bits, kRadixBits);
     const long long per_rec = (long long)sizeof(dsmc_observables) * n_traps;
-    x->obs_cap = (int)std::max<long long>(16, std::min<long long>(65536, (64LL << 20) / per_rec));
+    x->obs_cap = c->obs_capacity > 0
+                     ? c->obs_capacity
+                     : (int)std::max<long long>(16, std::min<long long>(65536, (256LL << 20) / per_rec));
 
     const int N = x->n_cap;
     x->d_traps = x->alloc<TrapDev>(n_traps, false);
@@ -680,23 +684,24 @@ static void observe_now(dsmc_ctx *x, std::vector<dsmc_observables> &out)
     }
 }
 
-// Initial per-cell state and labels (initSigvrmax, setUp.cu:287-292).
+// Initial per-cell state and labels (initSigvrmax, setUp.cu:287-292).  Every
+// transfer goes through the context's own (non-blocking) stream, then waits.
 static void reset_cells(dsmc_ctx *x)
 {
     std::vector<double> sv(x->n_keys, 0.0);
     for (int t = 0; t < x->T; ++t)
         for (int k = 0; k <= x->tdev[t].n_cells; ++k) sv[x->key_base[t] + k] = x->tdev[t].sigvrmax0;
-    DSMC_HIP(hipMemcpyAsync(x->sigvrmax, sv.data(), sizeof(double) * sv.size(), hipMemcpyHostToDevice,
-                            x->stream));
-    DSMC_HIP(hipStreamSynchronize(x->stream));
     const long long zero = 0;
-    DSMC_HIP(hipMemcpy(x->step_dev, &zero, sizeof zero, hipMemcpyHostToDevice));
-    DSMC_HIP(hipMemset(x->coll_accum, 0, sizeof(unsigned long long) * x->T));
     int nn[2] = { x->n_cap, x->n_cap };
-    DSMC_HIP(hipMemcpy(x->n_dev, nn, sizeof nn, hipMemcpyHostToDevice));
+    hipStream_t st = x->stream;
+    DSMC_HIP(hipMemcpyAsync(x->sigvrmax, sv.data(), sizeof(double) * sv.size(), hipMemcpyHostToDevice, st));
+    DSMC_HIP(hipMemcpyAsync(x->step_dev, &zero, sizeof zero, hipMemcpyHostToDevice, st));
+    DSMC_HIP(hipMemsetAsync(x->coll_accum, 0, sizeof(unsigned long long) * x->T, st));
+    DSMC_HIP(hipMemcpyAsync(x->n_dev, nn, sizeof nn, hipMemcpyHostToDevice, st));
     if (x->fp32)
-        DSMC_HIP(hipMemcpy(x->trap_start, x->atom_begin.data(), sizeof(int) * (x->T + 1),
-                           hipMemcpyHostToDevice));
+        DSMC_HIP(hipMemcpyAsync(x->trap_start, x->atom_begin.data(), sizeof(int) * (x->T + 1),
+                                hipMemcpyHostToDevice, st));
+    DSMC_HIP(hipStreamSynchronize(st));
     x->step = 0;
     x->coll_steps = 0;
     x->cur = 0;
@@ -944,33 +949,38 @@ dsmc_status dsmc_set_state(dsmc_ctx *ctx, const double *pos, const double *vel, 
     for (size_t i = 0; i < N; ++i) ids[i] = (int)i;
     if (!ctx->fp32) {
         const PlainPtrs &s = ctx->ps[0];
-        std::vector<double> col(N);
-        for (int c = 0; c < 3; ++c) {
-            for (size_t i = 0; i < N; ++i) col[i] = pos[3 * i + c];
-            DSMC_HIP(hipMemcpy(s.d[c], col.data(), N * 8, hipMemcpyHostToDevice));
-            for (size_t i = 0; i < N; ++i) col[i] = vel[3 * i + c];
-            DSMC_HIP(hipMemcpy(s.d[3 + c], col.data(), N * 8, hipMemcpyHostToDevice));
-        }
-        DSMC_HIP(hipMemcpy(s.id, ids.data(), N * 4, hipMemcpyHostToDevice));
-        DSMC_HIP(hipMemset(s.up, 1, N));
+        std::vector<double> cols(6 * N);
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < N; ++i) {
+                cols[c * N + i] = pos[3 * i + c];
+                cols[(3 + c) * N + i] = vel[3 * i + c];
+            }
+        for (int c = 0; c < 6; ++c)
+            DSMC_HIP(hipMemcpyAsync(s.d[c], cols.data() + c * N, N * 8, hipMemcpyHostToDevice,
+                                    ctx->stream));
+        DSMC_HIP(hipMemcpyAsync(s.id, ids.data(), N * 4, hipMemcpyHostToDevice, ctx->stream));
+        DSMC_HIP(hipMemsetAsync(s.up, 1, N, ctx->stream));
         launch_accel_plain(ctx->tdev[0], s, n, ctx->stream, &ctx->prof);
         plain_index(ctx);
     } else {
         const FusedPtrs &s = ctx->fs[0];
-        std::vector<float> hi(N), lo(N);
+        std::vector<float> hi(6 * N), lo(6 * N);
         for (int c = 0; c < 3; ++c) {
             for (int which = 0; which < 2; ++which) {
                 const double *src = which ? vel : pos;
+                float *h = hi.data() + (2 * c + which) * N, *l = lo.data() + (2 * c + which) * N;
                 for (size_t i = 0; i < N; ++i) {
                     const double v = src[3 * i + c];
-                    hi[i] = (float)v;
-                    lo[i] = (float)(v - (double)hi[i]);
+                    h[i] = (float)v;
+                    l[i] = (float)(v - (double)h[i]);
                 }
-                DSMC_HIP(hipMemcpy(which ? s.v[c] : s.p[c], hi.data(), N * 4, hipMemcpyHostToDevice));
-                DSMC_HIP(hipMemcpy(which ? s.cv[c] : s.cp[c], lo.data(), N * 4, hipMemcpyHostToDevice));
+                DSMC_HIP(hipMemcpyAsync(which ? s.v[c] : s.p[c], h, N * 4, hipMemcpyHostToDevice,
+                                        ctx->stream));
+                DSMC_HIP(hipMemcpyAsync(which ? s.cv[c] : s.cp[c], l, N * 4, hipMemcpyHostToDevice,
+                                        ctx->stream));
             }
         }
-        DSMC_HIP(hipMemcpy(s.id, ids.data(), N * 4, hipMemcpyHostToDevice));
+        DSMC_HIP(hipMemcpyAsync(s.id, ids.data(), N * 4, hipMemcpyHostToDevice, ctx->stream));
         fused_index(ctx);
     }
     DSMC_HIP(hipStreamSynchronize(ctx->stream));
@@ -985,18 +995,22 @@ dsmc_status dsmc_set_spin(dsmc_ctx *ctx, const double *spin, int n, int componen
     const size_t N = (size_t)n;
     if (!ctx->fp32) {
         if (components != 4) fail(DSMC_EINVAL, "the FP64 form stores spinors (4 components)");
-        std::vector<double> col(N);
+        std::vector<double> col(4 * N);
         for (int c = 0; c < 4; ++c) {
-            for (size_t i = 0; i < N; ++i) col[i] = spin[4 * i + c];
-            DSMC_HIP(hipMemcpy(ctx->ps[ctx->cur].psi[c], col.data(), N * 8, hipMemcpyHostToDevice));
+            for (size_t i = 0; i < N; ++i) col[c * N + i] = spin[4 * i + c];
+            DSMC_HIP(hipMemcpyAsync(ctx->ps[ctx->cur].psi[c], col.data() + c * N, N * 8,
+                                    hipMemcpyHostToDevice, ctx->stream));
         }
+        DSMC_HIP(hipStreamSynchronize(ctx->stream));
     } else {
         if (components != 3) fail(DSMC_EINVAL, "the FP32 form stores Bloch vectors (3 components)");
-        std::vector<float> col(N);
+        std::vector<float> col(3 * N);
         for (int c = 0; c < 3; ++c) {
-            for (size_t i = 0; i < N; ++i) col[i] = (float)spin[3 * i + c];
-            DSMC_HIP(hipMemcpy(ctx->fs[ctx->cur].s[c], col.data(), N * 4, hipMemcpyHostToDevice));
+            for (size_t i = 0; i < N; ++i) col[c * N + i] = (float)spin[3 * i + c];
+            DSMC_HIP(hipMemcpyAsync(ctx->fs[ctx->cur].s[c], col.data() + c * N, N * 4,
+                                    hipMemcpyHostToDevice, ctx->stream));
         }
+        DSMC_HIP(hipStreamSynchronize(ctx->stream));
     }
     DSMC_API_END(ctx)
 }

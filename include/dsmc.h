@@ -84,7 +84,8 @@ typedef struct dsmc_config {
     double gs;              /* spin precession factor (d_gs)                      */
     unsigned long long seed;/* Philox key                                         */
     int    block_size;      /* per-atom kernels; 0 = default                      */
-    int    reserved[7];
+    int    obs_capacity;    /* steps kept in the observables ring; 0 = default    */
+    int    reserved[6];
 } dsmc_config;
 
 /* FP64 observables of one trap after one motion step. */

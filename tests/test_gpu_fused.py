@@ -1,0 +1,186 @@
+"""GPU checks of the FP32 Kahan form (single and batched traps).
+
+Exact checks: FP64 cell ids of the stored positions, the sort permutation
+(numpy stable argsort), the occupied-cell table, HIP-graph replay against
+eager launches (bitwise), and a trap inside a batch against the same trap run
+alone (bitwise: each trap has its own Philox stream and the work of different
+traps never mixes).  Tolerance checks against the FP64 path and the oracle:
+collision-free trajectories, energy conservation |dE/E| <= 1e-4 over 1e4
+steps, and the energy-cut survivor set.
+"""
+import numpy as np
+import pytest
+
+from cudadsmc_amd import Simulation, _abi, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def quad(n, nc, precision="fp32", **kw):
+    run = configs.Run(traps=[configs.quadrupole_trap(n, nc)], precision=precision, dt=1e-6,
+                      seed=3, name="fused-test")
+    return run.with_(**kw) if kw else run
+
+
+def by_label(sim, field, n=None):
+    ids = sim.copy_out(_abi.FIELD_ATOM_ID)
+    a = sim.copy_out(field)
+    out = np.full(((n or sim.run.n_atoms),) + a.shape[1:], np.nan)
+    out[ids] = a
+    return out
+
+
+def test_fused_index_sort_table_exact(orc):
+    run = quad(30000, 40)
+    print("fused index/sort/table", flush=True)
+    ref = orc.Reference(run.with_(precision="fp64"))
+    with Simulation(run) as sim:
+        sim.set_state(ref.s["pos"], ref.s["vel"])
+        pos = sim.copy_out(_abi.FIELD_POS)  # float + remainder, as FP64
+        keys = sim.copy_out(_abi.FIELD_CELL_ID)
+        expect = orc.index_atoms(ref.p, pos, np.arange(run.n_atoms, dtype=np.int32))
+        np.testing.assert_array_equal(keys, expect)
+        sim.sort_arrays()
+        order = np.argsort(expect, kind="stable")
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_PERM), order)
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_CELL_ID), expect[order])
+        sim.build_cell_table()
+        se = sim.copy_out(_abi.FIELD_CELL_START_END)
+        se_o, count, _ = orc.cell_table(expect[order], orc.ncells(ref.p))
+        occ = count > 0
+        np.testing.assert_array_equal(se[: len(count)][occ], se_o[occ])
+        ts = sim.copy_out(_abi.FIELD_TRAP_START)
+        assert list(ts) == [0, run.n_atoms]
+        sim.collide()
+        sim.move_atoms()
+
+
+def test_fused_motion_matches_fp64(orc):
+    """Collision-free motion (sigma = 0): FP32 Kahan vs the FP64 path."""
+    run = quad(50000, 40, sigma=0.0)
+    ref = orc.Reference(run.with_(precision="fp64"))
+    print("fused motion vs fp64, 2000 steps", flush=True)
+    with Simulation(run) as s32, Simulation(run.with_(precision="fp64")) as s64:
+        s32.set_state(ref.s["pos"], ref.s["vel"])
+        s64.set_state(ref.s["pos"], ref.s["vel"])
+        s32.step(2000)
+        s64.step(2000)
+        p32, p64 = by_label(s32, _abi.FIELD_POS), by_label(s64, _abi.FIELD_POS)
+        v32, v64 = by_label(s32, _abi.FIELD_VEL), by_label(s64, _abi.FIELD_VEL)
+    rth = run.traps[0].thermal_radius()
+    dx = np.abs(p32 - p64).max() / rth
+    dv = np.abs(v32 - v64).max() / np.abs(v64).max()
+    print(f"max |dx|/r_th = {dx:.3e}, max |dv|/v = {dv:.3e}", flush=True)
+    assert dx < 1e-6 and dv < 1e-5
+
+
+@pytest.mark.parametrize("n,nc,steps", [(1000000, 100, 10000)])
+def test_fused_energy_conservation(n, nc, steps):
+    """|dE/E| <= 1e-4 over 1e4 steps with collisions on (N = 1e6, 100^3 cells)."""
+    run = configs.quadrupole_1e6("fp32")
+    print(f"energy conservation N={n} {nc}^3 {steps} steps", flush=True)
+    with Simulation(run) as sim:
+        sim.init_distribution()
+        e0 = sim.observe()[0]
+        E0 = e0.ek + e0.ep
+        drift = 0.0
+        coll = 0
+        for chunk in range(steps // 1000):
+            sim.step(1000)
+            log = sim.obs_log()[-1000:, 0]
+            E = log["ek"] + log["ep"]
+            drift = max(drift, float(np.max(np.abs(E - E0)) / E0))
+            coll += int(log["collisions"].sum())
+        print(f"max |dE/E| = {drift:.3e}, collisions = {coll}", flush=True)
+    assert drift <= 1e-4
+    assert coll > 0
+
+
+def test_graph_replay_equals_eager():
+    run = quad(40000, 30)
+    with Simulation(run) as a, Simulation(run.with_(use_graph=False)) as b:
+        a.init_distribution()
+        b.init_distribution()
+        a.step(51)
+        b.step(51)
+        for f in (_abi.FIELD_POS, _abi.FIELD_VEL, _abi.FIELD_ATOM_ID):
+            np.testing.assert_array_equal(a.copy_out(f), b.copy_out(f))
+        la, lb = a.obs_log(), b.obs_log()
+        np.testing.assert_array_equal(la["collisions"], lb["collisions"])
+        assert int(la["collisions"].sum()) > 0
+
+
+def _batch_run():
+    traps = []
+    for n, dBdz, T in [(3000, 1.5, 30e-6), (20000, 4.0, 8e-6), (7000, 2.5, 15e-6)]:
+        t = configs.quadrupole_trap(n, configs.cells_for_trap(n), T=T, dBdz=dBdz, n_real=3e6)
+        t.eta_start = t.eta_end = 6.0
+        t.T_cut = T
+        traps.append(t)
+    return configs.Run(traps=traps, precision="fp32", energy_cut=True, dt=1e-6, seed=11,
+                       name="batch-test")
+
+
+def test_batched_trap_equals_standalone():
+    run = _batch_run()
+    print("batched vs standalone", flush=True)
+    with Simulation(run) as sim:
+        sim.init_distribution()
+        sim.step(60)
+        ts = sim.copy_out(_abi.FIELD_TRAP_START)
+        ids = sim.copy_out(_abi.FIELD_ATOM_ID)
+        pos = sim.copy_out(_abi.FIELD_POS)
+        vel = sim.copy_out(_abi.FIELD_VEL)
+        log = sim.obs_log()
+    begin = np.cumsum([0] + [t.n_atoms for t in run.traps])
+    total_coll = 0
+    for t in range(len(run.traps)):
+        tr = run.traps[t]
+        tr.philox_stream = t
+        solo = configs.Run(traps=[tr], precision="fp32", energy_cut=True, dt=1e-6, seed=11)
+        with Simulation(solo) as s1:
+            s1.init_distribution()
+            s1.step(60)
+            ids1 = s1.copy_out(_abi.FIELD_ATOM_ID)
+            np.testing.assert_array_equal(ids[ts[t]:ts[t + 1]] - begin[t], ids1)
+            np.testing.assert_array_equal(pos[ts[t]:ts[t + 1]], s1.copy_out(_abi.FIELD_POS))
+            np.testing.assert_array_equal(vel[ts[t]:ts[t + 1]], s1.copy_out(_abi.FIELD_VEL))
+            l1 = s1.obs_log()
+            np.testing.assert_array_equal(log["collisions"][:, t], l1["collisions"][:, 0])
+            np.testing.assert_array_equal(log["n_evap"][:, t], l1["n_evap"][:, 0])
+            np.testing.assert_allclose(log["ek"][:, t], l1["ek"][:, 0], rtol=1e-12)
+            total_coll += int(l1["collisions"].sum())
+    assert total_coll > 0
+    assert int(log["n_evap"].sum()) > 0
+
+
+def test_energy_cut_survivors(orc):
+    """One step with a fixed cut: survivors = atoms whose FP64 energy after the
+    move is below eta kB T (oracle move of the same state), away from the
+    threshold by more than the FP32 trajectory error."""
+    run = quad(50000, 40, sigma=0.0, energy_cut=True)
+    T = run.traps[0].T
+    run.traps[0].eta_start = run.traps[0].eta_end = 5.0
+    run.traps[0].T_cut = T
+    ref = orc.Reference(run.with_(precision="fp64", energy_cut=False))
+    with Simulation(run) as sim:
+        sim.set_state(ref.s["pos"], ref.s["vel"])
+        sim.step(1)
+        sim.step(1)  # the second sort removes the atoms tagged by the first move
+        alive = set(sim.copy_out(_abi.FIELD_ATOM_ID).tolist())
+        nev = int(sim.obs_log()["n_evap"].sum())
+    ref.step(1)
+    p = ref.p
+    E = np.array([0.0])
+    ek = 0.5 * run.mass * np.sum(ref.s["vel"] ** 2, axis=1)
+    x, y, z = ref.s["pos"].T
+    ep = run.traps[0].mu * run.traps[0].dBdz * np.sqrt(x * x + y * y + 4.0 * z * z)
+    E = ek + ep
+    cut = 5.0 * configs.KB * T
+    far = np.abs(E - cut) > 1e-6 * cut
+    expect_alive = set(np.where((E <= cut) & far)[0].tolist())
+    expect_dead = set(np.where((E > cut) & far)[0].tolist())
+    assert expect_alive <= alive
+    assert not (expect_dead & alive)
+    assert nev >= len(expect_dead)
+    del p

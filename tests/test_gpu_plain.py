@@ -32,10 +32,10 @@ def start(orc, run):
 
 def by_label(sim, field):
     """GPU per-slot field reordered to atom-label order (the reference's
-    physical-index layout)."""
+    physical-index layout); rows of removed atoms are NaN."""
     ids = sim.copy_out(_abi.FIELD_ATOM_ID)
     a = sim.copy_out(field)
-    out = np.empty_like(a)
+    out = np.full((sim.run.n_atoms,) + a.shape[1:], np.nan)
     out[ids] = a
     return out, ids
 
@@ -131,10 +131,11 @@ def test_reference_run_spin_and_evaporation(orc):
         assert len(ids) < run.n_atoms
         pos, _ = by_label(sim, _abi.FIELD_POS)
         alive = ref.s["atom_id"]
-        np.testing.assert_allclose(pos[alive], ref.s["pos"][alive], rtol=0, atol=1e-18)
-        spin = sim.copy_out(_abi.FIELD_SPIN)
-        sp = np.empty_like(spin)
-        sp[ids] = spin
+        np.testing.assert_array_equal(pos[alive], ref.s["pos"][alive])
+        ev = sim.copy_out(_abi.FIELD_EVAP_POS)
+        lost = np.setdiff1d(np.arange(run.n_atoms), alive)
+        np.testing.assert_array_equal(ev[lost], ref.evap_pos[lost])
+        sp, _ = by_label(sim, _abi.FIELD_SPIN)
         np.testing.assert_allclose(sp[alive, :2], psi_u[alive], rtol=0, atol=1e-13)
         np.testing.assert_allclose(sp[alive, 2:], psi_d[alive], rtol=0, atol=1e-13)
 
