@@ -148,6 +148,8 @@ struct dsmc_ctx {
     uint32_t *cell_start = nullptr, *cell_end = nullptr;
     int *trap_start = nullptr;
     double *acc = nullptr, *acc2 = nullptr;
+    ObsRecords rec{};
+    int max_trap_atoms = 0;
     float *evap_pos_f = nullptr, *evap_vel_f = nullptr;
     // graph of two FP32 steps (parity 0 then 1)
     hipGraph_t graph = nullptr;
@@ -368,6 +370,13 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
         DSMC_HIP(hipMemcpyAsync(x->trap_start, x->atom_begin.data(), sizeof(int) * (n_traps + 1),
                                 hipMemcpyHostToDevice, x->stream));
         x->acc = x->alloc<double>((size_t)n_traps * kAccK);
+        x->rec.stride = obs_record_stride(N);
+        x->rec.trap = x->alloc<int>((size_t)x->rec.stride);
+        x->rec.sum = x->alloc<double>((size_t)x->rec.stride * kAccK);
+        x->rec.acc_over = x->acc;
+        x->rec.lvl2 = x->alloc<double>((size_t)n_traps * kMaxFinalizeP * kAccK);
+        x->rec.cnt = x->alloc<unsigned int>((size_t)n_traps);
+        for (int t = 0; t < n_traps; ++t) x->max_trap_atoms = std::max(x->max_trap_atoms, traps[t].n_atoms);
         x->acc2 = x->alloc<double>((size_t)n_traps * kAccK);
         x->evap_pos_f = x->alloc<float>(3 * (size_t)N);
         x->evap_vel_f = x->alloc<float>(3 * (size_t)N);
@@ -473,7 +482,7 @@ static void plain_compact(dsmc_ctx *x)
 static void plain_observe(dsmc_ctx *x, dsmc_observables *slot, long long step)
 {
     launch_observe_plain(x->tdev[0], x->cfg.mass, x->ps[x->cur], x->n_dev, x->n_cap, x->partials,
-                         x->done, slot, x->coll_accum, x->evap_accum, step, x->stream, &x->prof);
+                         slot, x->coll_accum, x->evap_accum, step, x->stream, &x->prof);
 }
 
 static void plain_step(dsmc_ctx *x)
@@ -538,14 +547,14 @@ static void fused_table(dsmc_ctx *x)
     tp.n_traps = x->T;
     tp.dead_key = x->dead_key;
     launch_table_fused(x->sorted_key, x->d_key_base, tp, x->n_dev + x->cur, x->n_dev + (x->cur ^ 1),
-                       x->cell_start, x->cell_end, x->trap_start, x->n_cap, x->stream, &x->prof);
+                       x->trap_start, x->n_cap, x->stream, &x->prof);
 }
 
 static void fused_collide(dsmc_ctx *x)
 {
     launch_collide_fused(x->d_traps, x->d_key_base, x->T, collide_params(x), x->sorted_key,
-                         x->perm, x->n_dev + (x->cur ^ 1), x->cell_end, x->fs[x->cur], x->sigvrmax,
-                         x->coll_accum, x->step_dev, x->dead_key, x->n_cap, x->stream, &x->prof);
+                         x->perm, x->n_dev + (x->cur ^ 1), x->fs[x->cur], x->sigvrmax,
+                         x->coll_accum, x->step_dev, x->n_cap, x->stream, &x->prof);
 }
 
 static void fused_move(dsmc_ctx *x)
@@ -554,8 +563,10 @@ static void fused_move(dsmc_ctx *x)
     const int *n = x->have_sort ? x->n_dev + (x->cur ^ 1) : x->n_dev + x->cur;
     launch_move_fused(x->block, x->d_traps, move_params(x), x->fs[x->cur], x->fs[x->cur ^ 1],
                       x->have_sort ? x->perm : nullptr, x->trap_start, n, x->key_a,
-                      x->evap_pos_f, x->evap_vel_f, x->acc, x->coll_accum, x->done, x->step_dev,
-                      x->obs_log, x->n_cap, x->stream, &x->prof);
+                      x->evap_pos_f, x->evap_vel_f, x->rec, x->step_dev, x->n_cap, x->stream,
+                      &x->prof);
+    launch_obs_finalize(x->T, x->max_trap_atoms, x->block, x->trap_start, x->rec, x->coll_accum,
+                        x->done, x->step_dev, x->obs_log, x->obs_cap, x->stream, &x->prof);
     if (!x->have_sort)  // keep the count with the state it describes
         DSMC_HIP(hipMemcpyAsync(x->n_dev + (x->cur ^ 1), x->n_dev + x->cur, sizeof(int),
                                 hipMemcpyDeviceToDevice, x->stream));
@@ -827,6 +838,12 @@ static void copy_field(dsmc_ctx *x, int field, void *host, size_t bytes)
             return;
         }
         case DSMC_FIELD_CELL_START_END: {
+            if (!x->have_sort) fail(DSMC_ESTATE, "the cell table exists between sort and move");
+            DSMC_HIP(hipMemsetAsync(x->cell_start, 0xFF, sizeof(uint32_t) * x->n_keys, x->stream));
+            DSMC_HIP(hipMemsetAsync(x->cell_end, 0, sizeof(uint32_t) * x->n_keys, x->stream));
+            launch_cell_range_fused(x->sorted_key, x->n_dev + (x->cur ^ 1), x->dead_key, x->cell_start,
+                                    x->cell_end, x->n_cap, x->stream);
+            DSMC_HIP(hipStreamSynchronize(x->stream));
             std::vector<uint32_t> a = d2h(x->cell_start, (size_t)x->n_keys);
             std::vector<uint32_t> b = d2h(x->cell_end, (size_t)x->n_keys);
             int *hi_ = (int *)host;
@@ -1264,9 +1281,8 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
             for (int k = 0; k < 5; ++k)
                 launch_move_fused(bs, ctx->d_traps, mp, ctx->fs[ctx->cur], ctx->fs[ctx->cur ^ 1],
                                   nullptr, ctx->trap_start, ctx->n_dev + ctx->cur, ctx->key_b,
-                                  ctx->evap_pos_f, ctx->evap_vel_f, ctx->acc, ctx->coll_accum,
-                                  ctx->done, ctx->step_dev, ctx->obs_log, ctx->n_cap, ctx->stream,
-                                  nullptr);
+                                  ctx->evap_pos_f, ctx->evap_vel_f, ctx->rec, ctx->step_dev,
+                                  ctx->n_cap, ctx->stream, nullptr);
             DSMC_HIP(hipEventRecord(e, ctx->stream));
             DSMC_HIP(hipEventSynchronize(e));
             DSMC_HIP(hipEventElapsedTime(&f, b, e));
@@ -1278,6 +1294,7 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
     (void)hipEventDestroy(e);
     DSMC_HIP(hipMemcpyAsync(ctx->step_dev, ctx->step_save, sizeof(long long),
                             hipMemcpyDeviceToDevice, ctx->stream));
+    DSMC_HIP(hipMemsetAsync(ctx->acc, 0, sizeof(double) * kAccK * ctx->T, ctx->stream));
     DSMC_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->block != best_bs && ctx->graph_exec) {
         (void)hipGraphExecDestroy(ctx->graph_exec);

@@ -122,30 +122,50 @@ __global__ __launch_bounds__(kFBlock) void k_table_fused(const uint32_t *__restr
                                                          TableParams tp,
                                                          const int *__restrict__ n_in_dev,
                                                          int *__restrict__ n_out_dev,
-                                                         uint32_t *__restrict__ cell_start,
-                                                         uint32_t *__restrict__ cell_end,
                                                          int *__restrict__ trap_start)
 {
     extern __shared__ int kb[];
+    __shared__ int blk_traps[2];
     const int T = tp.n_traps;
     load_table(kb, key_base, T + 1);
     const int n = *n_in_dev;
-    const int i = blockIdx.x * kFBlock + threadIdx.x;
-    if (i > n) return;
-    uint32_t k = tp.dead_key;
-    if (i < n) {
-        k = key[i];
-        if (k != tp.dead_key) {
-            if (i == 0 || key[i - 1] != k) cell_start[k] = (uint32_t)i;
-            if (i == n - 1 || key[i + 1] != k) cell_end[k] = (uint32_t)(i + 1);
-        }
+    const int b0 = blockIdx.x * kFBlock;
+    if (b0 > n) return;
+    // Fast path: a block whose keys all lie in one trap holds no trap boundary
+    // except possibly at its first slot.
+    if (threadIdx.x == 0) {
+        const int last = min(n, b0 + kFBlock) - 1;
+        blk_traps[0] = b0 < n ? last_le(kb, T + 1, (int)key[b0]) : T;
+        blk_traps[1] = last >= b0 ? last_le(kb, T + 1, (int)key[last]) : T;
     }
+    __syncthreads();
+    const int i = b0 + threadIdx.x;
+    if (i > n) return;
+    if (i != b0 && i != n && blk_traps[0] == blk_traps[1]) return;
+    const uint32_t k = (i < n) ? key[i] : tp.dead_key;
     const int cur = (i == n) ? T : last_le(kb, T + 1, (int)k);
     const int prev = (i == 0) ? -1 : last_le(kb, T + 1, (int)key[i - 1]);
     if (cur != prev) {
         for (int t = prev + 1; t <= cur; ++t) trap_start[t] = i;
         if (prev < T && cur == T) *n_out_dev = i;
     }
+}
+
+// Start and end slot of every occupied key (on demand: the output surface of
+// the cell table; the step itself does not need it).
+__global__ __launch_bounds__(kFBlock) void k_cell_range_fused(const uint32_t *__restrict__ key,
+                                                              const int *__restrict__ n_dev,
+                                                              uint32_t dead_key,
+                                                              uint32_t *__restrict__ cell_start,
+                                                              uint32_t *__restrict__ cell_end)
+{
+    const int n = *n_dev;
+    const int i = blockIdx.x * kFBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = key[i];
+    if (k == dead_key) return;
+    if (i == 0 || key[i - 1] != k) cell_start[k] = (uint32_t)i;
+    if (i == n - 1 || key[i + 1] != k) cell_end[k] = (uint32_t)(i + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -155,25 +175,22 @@ __global__ __launch_bounds__(kFBlock) void k_table_fused(const uint32_t *__restr
 // from value + remainder and stores the result back as float + remainder, so
 // momentum and energy are conserved to FP64 rounding.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kFBlock) void k_collide_fused(
-    const TrapDev *__restrict__ traps, const int *__restrict__ key_base, int T, CollideParams cp,
-    const uint32_t *__restrict__ key, const uint32_t *__restrict__ perm,
-    const int *__restrict__ n_dev, const uint32_t *__restrict__ cell_end, FusedPtrs s,
-    double *__restrict__ sigvrmax, unsigned long long *__restrict__ coll_accum,
-    const long long *__restrict__ step_dev, uint32_t dead_key)
+// Trials of one cell whose first sorted slot is i (collisions.cu:306-441).
+__device__ __forceinline__ void collide_cell(const TrapDev *__restrict__ traps, const int *kb, int T,
+                                             const CollideParams &cp,
+                                             const uint32_t *__restrict__ key,
+                                             const uint32_t *__restrict__ perm,
+                                             const FusedPtrs &s, double *__restrict__ sigvrmax,
+                                             unsigned long long *__restrict__ coll_accum,
+                                             const long long *__restrict__ step_dev, int i, int end)
 {
-    extern __shared__ int kb[];
-    load_table(kb, key_base, T + 1);
-    const int i = blockIdx.x * kFBlock + threadIdx.x;
-    if (i >= *n_dev) return;
+    const int n = end - i;
+    if (n < 2) return;
     const uint32_t k = key[i];
-    if (k == dead_key || (i > 0 && key[i - 1] == k)) return;
     const int t = last_le(kb, T + 1, (int)k);
     const TrapDev &tr = traps[t];
     const uint32_t cell = k - (uint32_t)tr.key_base;
     if ((int)cell >= tr.n_cells) return;  // overflow cell: no collisions
-    const int n = (int)cell_end[k] - i;
-    if (n < 2) return;
     const uint32_t step = (uint32_t)*step_dev;
     const double lpc = cp.lpc;
     double smax = sigvrmax[k];
@@ -237,6 +254,51 @@ __global__ __launch_bounds__(kFBlock) void k_collide_fused(
     if (count) atomicAdd(&coll_accum[t], (unsigned long long)count);
 }
 
+constexpr int kCollItems = 4;  // slots per lane scanned for cell heads
+
+__global__ __launch_bounds__(kFBlock) void k_collide_fused(
+    const TrapDev *__restrict__ traps, const int *__restrict__ key_base, int T, CollideParams cp,
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ perm,
+    const int *__restrict__ n_dev, FusedPtrs s, double *__restrict__ sigvrmax,
+    unsigned long long *__restrict__ coll_accum, const long long *__restrict__ step_dev)
+{
+    extern __shared__ int kb[];
+    __shared__ int heads[kFBlock / kWave][kWave * kCollItems + 1];
+    load_table(kb, key_base, T + 1);
+    const int n = *n_dev;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int chunk0 = (blockIdx.x * (kFBlock / kWave) + w) * (kWave * kCollItems);
+    const int chunk1 = min(n, chunk0 + kWave * kCollItems);
+    // Each wave compacts the first slots of the cells in its chunk (ballot +
+    // popcount) so that every lane then works on a different cell; a cell
+    // ends where the next one starts.
+    int nh = 0;
+    for (int it = 0; it < kCollItems; ++it) {
+        const int i = chunk0 + it * kWave + lane;
+        bool head = false;
+        if (i < n) head = i == 0 || key[i - 1] != key[i];
+        const uint64_t bal = __ballot(head);
+        if (head) heads[w][nh + __popcll(bal & lanemask_lt())] = i;
+        nh += __popcll(bal);
+    }
+    if (lane == 0 && nh > 0) {
+        // end of the last cell of the chunk: gallop forward through the keys
+        const uint32_t k = key[heads[w][nh - 1]];
+        int lo = chunk1, step = 1;
+        while (lo + step - 1 < n && key[lo + step - 1] == k) { lo += step; step <<= 1; }
+        int hi = min(n, lo + step - 1);  // first slot with another key is in [lo, hi]
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (key[mid] == k) lo = mid + 1; else hi = mid;
+        }
+        heads[w][nh] = lo;
+    }
+    __syncthreads();
+    for (int h = lane; h < nh; h += kWave)
+        collide_cell(traps, kb, T, cp, key, perm, s, sigvrmax, coll_accum, step_dev, heads[w][h],
+                     heads[w][h + 1]);
+}
+
 // ---------------------------------------------------------------------------
 // Fused motion step: gather through the permutation, spin substeps,
 // Kahan-compensated velocity Verlet (moveAtoms.cu:51-91, acceleration
@@ -250,24 +312,27 @@ __global__ __launch_bounds__(BS) void k_move_fused(
     const TrapDev *__restrict__ traps, MoveParams mp, FusedPtrs A, FusedPtrs B,
     const uint32_t *__restrict__ perm, const int *__restrict__ trap_start,
     const int *__restrict__ n_dev, uint32_t *__restrict__ key_out, float *__restrict__ evap_pos,
-    float *__restrict__ evap_vel, double *__restrict__ acc,
-    unsigned long long *__restrict__ coll_accum, unsigned int *__restrict__ done,
-    long long *__restrict__ step_dev, dsmc_observables *__restrict__ obs_log)
+    float *__restrict__ evap_vel, int *__restrict__ rec_trap, double *__restrict__ rec_sum,
+    int rec_stride, double *__restrict__ acc_over, const long long *__restrict__ step_dev)
 {
     extern __shared__ int ts[];
     __shared__ double red[(BS / kWave) * kAccK];
-    __shared__ bool last;
     const int T = mp.n_traps;
     load_table(ts, trap_start, T + 1);
     const int n = *n_dev;
     const long long step = *step_dev;
     const int i = blockIdx.x * BS + threadIdx.x;
+    const int b0 = blockIdx.x * BS;
+    // traps covered by this block: usually one, so no per-thread search
+    const int t_first = b0 < n ? last_le(ts, T, b0) : 0;
+    const int t_last = b0 < n ? last_le(ts, T, min(n, b0 + BS) - 1) : 0;
     double vals[kAccK];
 #pragma unroll
     for (int q = 0; q < kAccK; ++q) vals[q] = 0.0;
     int t = -1;
     if (i < n) {
-        t = last_le(ts, T, i);
+        t = t_first;
+        while (t < t_last && ts[t + 1] <= i) ++t;
         const TrapDev &tr = traps[t];
         const uint32_t j = perm ? perm[i] : (uint32_t)i;
         float x = A.p[0][j], y = A.p[1][j], z = A.p[2][j];
@@ -363,53 +428,99 @@ __global__ __launch_bounds__(BS) void k_move_fused(
         key_out[i] = kout;
     }
 
-    // Per-trap block sums (atoms are trap-major, so a block spans a few traps).
-    const int b0 = blockIdx.x * BS;
+    // Per-trap block sums.  Atoms are trap-major, so a block covers traps
+    // t_first..t_last, usually one.  The sums of t_first go to this block's
+    // record (structure of arrays, read coalesced by k_obs_finalize); those of
+    // later traps go to acc_over by atomics.  Only the block holding a trap's
+    // first slot can do that, so each accumulator receives at most one term
+    // and the totals do not depend on the order of execution.
     if (b0 < n) {
-        const int t_first = last_le(ts, T, b0);
-        const int t_last = last_le(ts, T, min(n, b0 + BS) - 1);
-        for (int tt = t_first; tt <= t_last; ++tt) {
+        const int nseg = t_last - t_first + 1;
+        for (int sg = 0; sg < nseg; ++sg) {
+            const int tt = t_first + sg;
             double v[kAccK];
 #pragma unroll
             for (int q = 0; q < kAccK; ++q) v[q] = (t == tt) ? vals[q] : 0.0;
             block_sum<BS, kAccK>(v, red);
             if (threadIdx.x == 0) {
+                if (sg == 0) {
 #pragma unroll
-                for (int q = 0; q < kAccK; ++q)
-                    if (v[q] != 0.0) atomicAdd(&acc[tt * kAccK + q], v[q]);
+                    for (int q = 0; q < kAccK; ++q) rec_sum[(size_t)q * rec_stride + blockIdx.x] = v[q];
+                    rec_trap[blockIdx.x] = t_first;
+                } else {
+                    for (int q = 0; q < kAccK; ++q)
+                        if (v[q] != 0.0) atomicAdd(&acc_over[tt * kAccK + q], v[q]);
+                }
             }
         }
     }
+}
 
+// Grid (P, T): block (p, t) adds the records of every P-th move block of trap
+// t (fixed order), the last of the P blocks combines their results in order,
+// adds the boundary term, appends the trap's observables to the log and
+// clears the per-step accumulators; the last trap advances the step.
+__global__ __launch_bounds__(256) void k_obs_finalize(
+    int T, int P, int move_block, const int *__restrict__ trap_start,
+    const int *__restrict__ rec_trap, const double *__restrict__ rec_sum, int rec_stride,
+    double *__restrict__ lvl2, unsigned int *__restrict__ cnt, double *__restrict__ acc_over,
+    unsigned long long *__restrict__ coll_accum, unsigned int *__restrict__ done,
+    long long *__restrict__ step_dev, dsmc_observables *__restrict__ obs_log, int obs_cap)
+{
+    __shared__ double red[(256 / kWave) * kAccK];
+    __shared__ bool last;
+    const int p = blockIdx.x, t = blockIdx.y;
+    const long long step = *step_dev;
+    const int s0 = trap_start[t], s1 = trap_start[t + 1];
+    double a[kAccK];
+#pragma unroll
+    for (int q = 0; q < kAccK; ++q) a[q] = 0.0;
+    if (s1 > s0) {
+        const int bA = s0 / move_block, bB = (s1 - 1) / move_block;
+        for (int b = bA + p * 256 + (int)threadIdx.x; b <= bB; b += P * 256) {
+            if (rec_trap[b] == t) {
+#pragma unroll
+                for (int q = 0; q < kAccK; ++q) a[q] += rec_sum[(size_t)q * rec_stride + b];
+            }
+        }
+    }
+    block_sum<256, kAccK>(a, red);
     if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < kAccK; ++q)
+            __hip_atomic_store(&lvl2[((size_t)t * P + p) * kAccK + q], a[q], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
-        last = atomicAdd(done, 1u) == gridDim.x - 1;
+        last = atomicAdd(&cnt[t], 1u) == (unsigned)(P - 1);
     }
     __syncthreads();
-    if (!last) return;
+    if (!last || threadIdx.x != 0) return;
     __threadfence();
-    const long long slot = step % mp.obs_cap;
-    for (int tt = threadIdx.x; tt < T; tt += BS) {
-        double a[kAccK];
+    double f[kAccK];
 #pragma unroll
-        for (int q = 0; q < kAccK; ++q) {
-            a[q] = __hip_atomic_load(&acc[tt * kAccK + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&acc[tt * kAccK + q], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        const unsigned long long c =
-            __hip_atomic_load(&coll_accum[tt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&coll_accum[tt], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        dsmc_observables &o = obs_log[slot * T + tt];
-        o.ek = a[kAccEk];
-        o.ep = a[kAccEp];
-        o.v2[0] = a[kAccV2x]; o.v2[1] = a[kAccV2y]; o.v2[2] = a[kAccV2z];
-        o.e_evap = a[kAccEevap];
-        o.n_alive = (long long)a[kAccN] + (long long)a[kAccNevap];
-        o.n_evap = (long long)a[kAccNevap];
-        o.collisions = (long long)c;
-        o.step = step;
+    for (int q = 0; q < kAccK; ++q) f[q] = 0.0;
+    for (int pp = 0; pp < P; ++pp)
+        for (int q = 0; q < kAccK; ++q)
+            f[q] += __hip_atomic_load(&lvl2[((size_t)t * P + pp) * kAccK + q], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+    for (int q = 0; q < kAccK; ++q) {
+        f[q] += acc_over[t * kAccK + q];
+        acc_over[t * kAccK + q] = 0.0;
     }
-    if (threadIdx.x == 0) {
+    cnt[t] = 0u;
+    const unsigned long long c = coll_accum[t];
+    coll_accum[t] = 0ull;
+    dsmc_observables &o = obs_log[(step % obs_cap) * T + t];
+    o.ek = f[kAccEk];
+    o.ep = f[kAccEp];
+    o.v2[0] = f[kAccV2x]; o.v2[1] = f[kAccV2y]; o.v2[2] = f[kAccV2z];
+    o.e_evap = f[kAccEevap];
+    o.n_alive = (long long)f[kAccN] + (long long)f[kAccNevap];
+    o.n_evap = (long long)f[kAccNevap];
+    o.collisions = (long long)c;
+    o.step = step;
+    __threadfence();
+    if (atomicAdd(done, 1u) == (unsigned)(T - 1)) {
         *done = 0u;
         *step_dev = step + 1;
     }
@@ -463,29 +574,36 @@ void launch_index_fused(const TrapDev *traps, const int *trap_start, int n_traps
 }
 
 void launch_table_fused(const uint32_t *key, const int *key_base, const TableParams &tp,
-                        const int *n_in, int *n_out, uint32_t *cell_start, uint32_t *cell_end,
-                        int *trap_start, int n_cap, hipStream_t st, Profiler *prof)
+                        const int *n_in, int *n_out, int *trap_start, int n_cap, hipStream_t st,
+                        Profiler *prof)
 {
     const size_t lds = sizeof(int) * (size_t)(tp.n_traps + 1);
     if (prof && prof->enabled) prof->begin(st, "table_fused", (double)n_cap * 4.0);
     hipLaunchKernelGGL(k_table_fused, dim3(fgrid((long long)n_cap + 1, kFBlock)), dim3(kFBlock),
-                       lds, st, key, key_base, tp, n_in, n_out, cell_start, cell_end, trap_start);
+                       lds, st, key, key_base, tp, n_in, n_out, trap_start);
     DSMC_HIP(hipGetLastError());
     if (prof && prof->enabled) prof->end(st);
 }
 
+void launch_cell_range_fused(const uint32_t *key, const int *n_dev, uint32_t dead_key,
+                             uint32_t *cell_start, uint32_t *cell_end, int n_cap, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cell_range_fused, dim3(fgrid(n_cap, kFBlock)), dim3(kFBlock), 0, st, key,
+                       n_dev, dead_key, cell_start, cell_end);
+    DSMC_HIP(hipGetLastError());
+}
+
 void launch_collide_fused(const TrapDev *traps, const int *key_base, int n_traps,
                           const CollideParams &cp, const uint32_t *key, const uint32_t *perm,
-                          const int *n_dev, const uint32_t *cell_end, const FusedPtrs &s,
-                          double *sigvrmax, unsigned long long *coll_accum,
-                          const long long *step_dev, uint32_t dead_key, int n_cap,
+                          const int *n_dev, const FusedPtrs &s, double *sigvrmax,
+                          unsigned long long *coll_accum, const long long *step_dev, int n_cap,
                           hipStream_t st, Profiler *prof)
 {
     const size_t lds = sizeof(int) * (size_t)(n_traps + 1);
-    if (prof && prof->enabled) prof->begin(st, "collide_fused", (double)n_cap * 8.0);
-    hipLaunchKernelGGL(k_collide_fused, dim3(fgrid(n_cap, kFBlock)), dim3(kFBlock), lds, st,
-                       traps, key_base, n_traps, cp, key, perm, n_dev, cell_end, s, sigvrmax,
-                       coll_accum, step_dev, dead_key);
+    if (prof && prof->enabled) prof->begin(st, "collide_fused", (double)n_cap * 4.0);
+    hipLaunchKernelGGL(k_collide_fused, dim3(fgrid(n_cap, kFBlock * kCollItems)), dim3(kFBlock), lds,
+                       st, traps, key_base, n_traps, cp, key, perm, n_dev, s, sigvrmax, coll_accum,
+                       step_dev);
     DSMC_HIP(hipGetLastError());
     if (prof && prof->enabled) prof->end(st);
 }
@@ -501,49 +619,56 @@ template <int BS>
 static void move_fused_bs(const TrapDev *traps, const MoveParams &mp, const FusedPtrs &src,
                           const FusedPtrs &dst, const uint32_t *perm, const int *trap_start,
                           const int *n_dev, uint32_t *key_out, float *evap_pos, float *evap_vel,
-                          double *acc, unsigned long long *coll_accum, unsigned int *done,
-                          long long *step_dev, dsmc_observables *obs_log, int n_cap,
+                          const ObsRecords &rec, const long long *step_dev, int n_cap,
                           hipStream_t st)
 {
     const size_t lds = sizeof(int) * (size_t)(mp.n_traps + 1);
     hipLaunchKernelGGL(k_move_fused<BS>, dim3(fgrid(n_cap, BS)), dim3(BS), lds, st, traps, mp, src,
-                       dst, perm, trap_start, n_dev, key_out, evap_pos, evap_vel, acc, coll_accum,
-                       done, step_dev, obs_log);
+                       dst, perm, trap_start, n_dev, key_out, evap_pos, evap_vel, rec.trap, rec.sum,
+                       rec.stride, rec.acc_over, step_dev);
 }
+
+int obs_record_stride(int n_cap) { return fgrid(n_cap, 64) + 1; }
 
 void launch_move_fused(int block, const TrapDev *traps, const MoveParams &mp, const FusedPtrs &src,
                        const FusedPtrs &dst, const uint32_t *perm, const int *trap_start,
                        const int *n_dev, uint32_t *key_out, float *evap_pos, float *evap_vel,
-                       double *acc, unsigned long long *coll_accum, unsigned int *done,
-                       long long *step_dev, dsmc_observables *obs_log, int n_cap,
-                       hipStream_t st, Profiler *prof)
+                       const ObsRecords &rec, const long long *step_dev, int n_cap, hipStream_t st,
+                       Profiler *prof)
 {
     if (prof && prof->enabled)
         prof->begin(st, "move_fused", (double)n_cap * move_fused_bytes_per_atom(mp.spin));
+#define DSMC_MOVE_CASE(BSV)                                                                     \
+    case BSV:                                                                                   \
+        move_fused_bs<BSV>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,     \
+                           evap_vel, rec, step_dev, n_cap, st);                                 \
+        break;
     switch (block) {
-    case 64:
-        move_fused_bs<64>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos, evap_vel,
-                          acc, coll_accum, done, step_dev, obs_log, n_cap, st);
-        break;
-    case 128:
-        move_fused_bs<128>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
-                           evap_vel, acc, coll_accum, done, step_dev, obs_log, n_cap, st);
-        break;
-    case 512:
-        move_fused_bs<512>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
-                           evap_vel, acc, coll_accum, done, step_dev, obs_log, n_cap, st);
-        break;
-    case 1024:
-        move_fused_bs<1024>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
-                            evap_vel, acc, coll_accum, done, step_dev, obs_log, n_cap, st);
-        break;
+        DSMC_MOVE_CASE(64)
+        DSMC_MOVE_CASE(128)
+        DSMC_MOVE_CASE(512)
+        DSMC_MOVE_CASE(1024)
     default:
         move_fused_bs<256>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
-                           evap_vel, acc, coll_accum, done, step_dev, obs_log, n_cap, st);
+                           evap_vel, rec, step_dev, n_cap, st);
         break;
     }
+#undef DSMC_MOVE_CASE
     DSMC_HIP(hipGetLastError());
     if (prof && prof->enabled) prof->end(st);
+}
+
+void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, const int *trap_start,
+                         const ObsRecords &rec, unsigned long long *coll_accum, unsigned int *done,
+                         long long *step_dev, dsmc_observables *obs_log, int obs_cap,
+                         hipStream_t st, Profiler *prof)
+{
+    const int blocks = div_up(max_trap_atoms > 0 ? max_trap_atoms : 1, move_block) + 1;
+    const int P = std::min(kMaxFinalizeP, div_up(blocks, 2048));
+    DSMC_LAUNCH(prof, "obs_finalize", (double)n_traps * 64.0, k_obs_finalize, dim3(P, n_traps), 256,
+                st, n_traps, P, move_block, trap_start, (const int *)rec.trap,
+                (const double *)rec.sum, rec.stride, rec.lvl2, rec.cnt, rec.acc_over, coll_accum,
+                done, step_dev, obs_log, obs_cap);
 }
 
 void launch_gather_fused(const FusedPtrs &dst, const FusedPtrs &src, const uint32_t *map,

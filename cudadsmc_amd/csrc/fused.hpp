@@ -53,20 +53,35 @@ void launch_index_fused(const TrapDev *traps, const int *trap_start, int n_traps
                         const FusedPtrs &s, uint32_t *key, const int *n_dev, int n_cap,
                         hipStream_t st, Profiler *prof);
 void launch_table_fused(const uint32_t *key, const int *key_base, const TableParams &tp,
-                        const int *n_in, int *n_out, uint32_t *cell_start, uint32_t *cell_end,
-                        int *trap_start, int n_cap, hipStream_t st, Profiler *prof);
+                        const int *n_in, int *n_out, int *trap_start, int n_cap, hipStream_t st,
+                        Profiler *prof);
+void launch_cell_range_fused(const uint32_t *key, const int *n_dev, uint32_t dead_key,
+                             uint32_t *cell_start, uint32_t *cell_end, int n_cap, hipStream_t st);
 void launch_collide_fused(const TrapDev *traps, const int *key_base, int n_traps,
                           const CollideParams &cp, const uint32_t *key, const uint32_t *perm,
-                          const int *n_dev, const uint32_t *cell_end, const FusedPtrs &s,
-                          double *sigvrmax, unsigned long long *coll_accum,
-                          const long long *step_dev, uint32_t dead_key, int n_cap,
+                          const int *n_dev, const FusedPtrs &s, double *sigvrmax,
+                          unsigned long long *coll_accum, const long long *step_dev, int n_cap,
                           hipStream_t st, Profiler *prof);
+// Per-block observable records of the fused move (see k_move_fused).
+constexpr int kMaxFinalizeP = 64;
+struct ObsRecords {
+    int    *trap;      // first trap of each move block            [stride]
+    double *sum;       // its sums, kAccK arrays of length stride   [kAccK * stride]
+    int     stride;
+    double *acc_over;  // boundary terms per trap                   [T * kAccK]
+    double *lvl2;      // finalize partials                         [T * kMaxFinalizeP * kAccK]
+    unsigned int *cnt; // finalize counters per trap                [T]
+};
+int  obs_record_stride(int n_cap);
 void launch_move_fused(int block, const TrapDev *traps, const MoveParams &mp, const FusedPtrs &src,
                        const FusedPtrs &dst, const uint32_t *perm, const int *trap_start,
                        const int *n_dev, uint32_t *key_out, float *evap_pos, float *evap_vel,
-                       double *acc, unsigned long long *coll_accum, unsigned int *done,
-                       long long *step_dev, dsmc_observables *obs_log, int n_cap,
-                       hipStream_t st, Profiler *prof);
+                       const ObsRecords &rec, const long long *step_dev, int n_cap, hipStream_t st,
+                       Profiler *prof);
+void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, const int *trap_start,
+                         const ObsRecords &rec, unsigned long long *coll_accum, unsigned int *done,
+                         long long *step_dev, dsmc_observables *obs_log, int obs_cap,
+                         hipStream_t st, Profiler *prof);
 void launch_gather_fused(const FusedPtrs &dst, const FusedPtrs &src, const uint32_t *map,
                          const int *n_dev, int n_cap, int spin, hipStream_t st, Profiler *prof);
 double move_fused_bytes_per_atom(int spin);

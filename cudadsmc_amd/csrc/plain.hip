@@ -266,14 +266,11 @@ __global__ __launch_bounds__(kBlock) void k_evap_tag_plain(TrapDev t, EvapParams
 // ---------------------------------------------------------------------------
 constexpr int kObsK = 5;  // ek, ep, vx^2, vy^2, vz^2
 
-__global__ __launch_bounds__(kBlock) void k_observe_plain(
-    TrapDev t, double mass, PlainPtrs s, const int *__restrict__ n_dev,
-    double *__restrict__ partials, unsigned int *__restrict__ done,
-    dsmc_observables *__restrict__ out, unsigned long long *__restrict__ coll_accum,
-    double *__restrict__ evap_accum, long long step)
+__global__ __launch_bounds__(kBlock) void k_observe_plain(TrapDev t, double mass, PlainPtrs s,
+                                                          const int *__restrict__ n_dev,
+                                                          double *__restrict__ partials)
 {
     __shared__ double tmp[(kBlock / kWave) * kObsK];
-    __shared__ bool last;
     const int n = *n_dev;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     double v[kObsK] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
@@ -288,14 +285,22 @@ __global__ __launch_bounds__(kBlock) void k_observe_plain(
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < kObsK; ++k) partials[(size_t)blockIdx.x * kObsK + k] = v[k];
-        __threadfence();
-        last = atomicAdd(done, 1u) == gridDim.x - 1;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+}
+
+// One block adds the block partials in a fixed order (reproducible) and
+// writes the log record; it also takes and clears the collision and
+// evaporation accumulators of the step.
+__global__ __launch_bounds__(kBlock) void k_observe_finalize_plain(
+    const int *__restrict__ n_dev, const double *__restrict__ partials, int nb,
+    dsmc_observables *__restrict__ out, unsigned long long *__restrict__ coll_accum,
+    double *__restrict__ evap_accum, long long step)
+{
+    __shared__ double tmp[(kBlock / kWave) * kObsK];
+    const int n = *n_dev;
+    const int used = (n + kBlock - 1) / kBlock;
     double acc[kObsK] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
+    for (int b = threadIdx.x; b < used && b < nb; b += kBlock) {
 #pragma unroll
         for (int k = 0; k < kObsK; ++k) acc[k] += partials[(size_t)b * kObsK + k];
     }
@@ -311,7 +316,6 @@ __global__ __launch_bounds__(kBlock) void k_observe_plain(
         out->step = step;
         evap_accum[0] = 0.0; evap_accum[1] = 0.0;
         *coll_accum = 0ull;
-        *done = 0u;
     }
 }
 
@@ -427,13 +431,15 @@ void launch_evap_tag_plain(const TrapDev &t, const EvapParams &ep, const PlainPt
 int observe_plain_blocks(int n_cap) { return grid_for(n_cap); }
 
 void launch_observe_plain(const TrapDev &t, double mass, const PlainPtrs &s, const int *n_dev,
-                          int n_cap, double *partials, unsigned int *done,
-                          dsmc_observables *out, unsigned long long *coll_accum,
-                          double *evap_accum, long long step, hipStream_t st, Profiler *prof)
+                          int n_cap, double *partials, dsmc_observables *out,
+                          unsigned long long *coll_accum, double *evap_accum, long long step,
+                          hipStream_t st, Profiler *prof)
 {
     DSMC_LAUNCH(prof, "observe_plain", (double)n_cap * 48.0, k_observe_plain, grid_for(n_cap),
-                kBlock, st, t, mass, s, n_dev, partials, done, out, coll_accum, evap_accum,
-                step);
+                kBlock, st, t, mass, s, n_dev, partials);
+    DSMC_LAUNCH(prof, "observe_finalize", (double)grid_for(n_cap) * 40.0, k_observe_finalize_plain,
+                1, kBlock, st, n_dev, (const double *)partials, grid_for(n_cap), out, coll_accum,
+                evap_accum, step);
 }
 
 void launch_init_plain(const TrapDev &t, const InitParams &ip, const PlainPtrs &s, int n,

@@ -62,9 +62,9 @@ void launch_evap_tag_plain(const TrapDev &t, const EvapParams &ep, const PlainPt
                            const int *n_dev, int n_cap, hipStream_t st, Profiler *prof);
 int  observe_plain_blocks(int n_cap);
 void launch_observe_plain(const TrapDev &t, double mass, const PlainPtrs &s, const int *n_dev,
-                          int n_cap, double *partials, unsigned int *done,
-                          dsmc_observables *out, unsigned long long *coll_accum,
-                          double *evap_accum, long long step, hipStream_t st, Profiler *prof);
+                          int n_cap, double *partials, dsmc_observables *out,
+                          unsigned long long *coll_accum, double *evap_accum, long long step,
+                          hipStream_t st, Profiler *prof);
 void launch_init_plain(const TrapDev &t, const InitParams &ip, const PlainPtrs &s, int n,
                        hipStream_t st, Profiler *prof);
 void launch_accel_plain(const TrapDev &t, const PlainPtrs &s, int n, hipStream_t st,
