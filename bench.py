@@ -98,6 +98,26 @@ def profile_leg(sim, steps):
     return out, n_mean
 
 
+def copy_calibration(mbytes=1024, repeats=20):
+    """Achieved GB/s of a streaming copy (read + write) of `mbytes` MB: the
+    practical bandwidth ceiling of this device, and the reference kernel for
+    scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    from cudadsmc_amd import load_library
+
+    lib = load_library()
+    nbytes = mbytes << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert lib.dsmc_device_alloc(C.byref(a), nbytes) == 0
+    assert lib.dsmc_device_alloc(C.byref(b), nbytes) == 0
+    ms = C.c_double()
+    st = lib.dsmc_stream_copy(b, a, nbytes, repeats, C.byref(ms))
+    lib.dsmc_device_free(a)
+    lib.dsmc_device_free(b)
+    assert st == 0
+    return dict(bytes=nbytes, ms=ms.value, GBs=2.0 * nbytes / (ms.value * 1e-3) / 1e9)
+
+
 def cpu_baseline(target_s=12.0):
     from cudadsmc_amd import configs
     from oracle import oracle as orc
@@ -126,6 +146,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--legs", default="batched,fp64,fp32")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only run the streaming-copy bandwidth calibration")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -136,6 +158,9 @@ def main():
     from cudadsmc_amd import configs, load_library
 
     load_library()
+    if args.copy_only:
+        print(json.dumps({"copy_calibration": copy_calibration()}), flush=True)
+        return
     legs = set(args.legs.split(","))
     result = {}
 

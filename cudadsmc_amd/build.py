@@ -1,9 +1,8 @@
-"""Build recipes: the HIP library, the C++ host driver and the CPU oracle.
+"""Build recipes of the product: the HIP library and the C++ host driver.
 
 `build_all()` compiles, in place:
   * cudadsmc_amd/libdsmc.so      -- HIP kernels + C ABI (hipcc, gfx950)
   * cudadsmc_amd/bin/cudadsmc    -- C++ host driver (CLI of main.cu:36-69)
-  * oracle/liboracle.so          -- CPU restatement (test infrastructure)
 Every HIP source is compiled with -ffp-contract=off (FP64 results reproducible
 by the CPU restatement) and without packed FP32 instructions.
 """
@@ -21,7 +20,6 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libdsmc.so"
 DRIVER = PKG / "bin" / "cudadsmc"
-ORACLE_DIR = ROOT / "oracle"
 
 HIP_SOURCES = ["primitives.hip", "plain.hip", "fused.hip", "context.hip"]
 HIPCC_FLAGS = [
@@ -80,13 +78,8 @@ def build_driver() -> Path:
     return DRIVER
 
 
-def build_oracle() -> Path:
-    _run(["make", "-s", "-C", str(ORACLE_DIR)])
-    return ORACLE_DIR / "liboracle.so"
-
-
 def build_all(verbose: bool = True) -> None:
-    steps = [("libdsmc.so", build_library), ("oracle", build_oracle), ("driver", build_driver)]
+    steps = [("libdsmc.so", build_library), ("driver", build_driver)]
     for name, fn in steps:
         out = fn()
         if verbose:

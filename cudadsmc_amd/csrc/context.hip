@@ -1379,6 +1379,27 @@ dsmc_status dsmc_philox(unsigned int *d_out, int n, const unsigned int ctr[4],
     PRIM_END
 }
 
+dsmc_status dsmc_stream_copy(void *d_dst, const void *d_src, size_t bytes, int repeats,
+                             double *ms)
+{
+    PRIM_BEGIN
+    if (!d_dst || !d_src || bytes % 16 || repeats < 1) fail(DSMC_EINVAL, "bad stream_copy arguments");
+    hipEvent_t b, e;
+    DSMC_HIP(hipEventCreate(&b));
+    DSMC_HIP(hipEventCreate(&e));
+    stream_copy(d_dst, d_src, bytes, nullptr);  // warm-up
+    DSMC_HIP(hipEventRecord(b, nullptr));
+    for (int r = 0; r < repeats; ++r) stream_copy(d_dst, d_src, bytes, nullptr);
+    DSMC_HIP(hipEventRecord(e, nullptr));
+    DSMC_HIP(hipEventSynchronize(e));
+    float f = 0.f;
+    DSMC_HIP(hipEventElapsedTime(&f, b, e));
+    (void)hipEventDestroy(b);
+    (void)hipEventDestroy(e);
+    if (ms) *ms = f / repeats;
+    PRIM_END
+}
+
 dsmc_status dsmc_device_alloc(void **d_ptr, size_t bytes)
 {
     PRIM_BEGIN

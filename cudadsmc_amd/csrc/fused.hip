@@ -195,8 +195,10 @@ __device__ __forceinline__ void collide_cell(const TrapDev *__restrict__ traps, 
     const double lpc = cp.lpc;
     double smax = sigvrmax[k];
     const double Mc = 0.5 * (n - 1) * n;
-    const double lambda = ceil(Mc * tr.alpha * lpc * cp.dt * smax / tr.cell_vol) / Mc;
-    const int Ncol = (int)(Mc * lambda);
+    const double X = Mc * tr.alpha * lpc * cp.dt * smax / tr.cell_vol;
+    const double lambda = ceil(X) / Mc;
+    // reference pair mode keeps collisions.cu:322's truncation of Mc * lambda
+    const int Ncol = cp.pair_mode ? (int)(Mc * lambda) : (int)ceil(X);
     const float pc = (float)(tr.alpha * lpc * cp.dt / tr.cell_vol * cp.sigma / lambda);
     int count = 0;
     for (int tr_i = 0; tr_i < Ncol; ++tr_i) {

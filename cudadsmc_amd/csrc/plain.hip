@@ -88,8 +88,10 @@ __global__ __launch_bounds__(kBlock) void k_collide_plain(
         const int n = prefix[cell + 1] - prefix[cell];
         if (n <= 1) continue;
         const double Mc = 0.5 * (n - 1) * n;
-        const double lambda = ceil(Mc * t.alpha * lpc * cp.dt * sigvrmax[cell] / cellVolume) / Mc;
-        const int Ncol = (int)(Mc * lambda);
+        const double X = Mc * t.alpha * lpc * cp.dt * sigvrmax[cell] / cellVolume;
+        const double lambda = ceil(X) / Mc;
+        // reference pair mode keeps collisions.cu:322's truncation of Mc * lambda
+        const int Ncol = cp.pair_mode ? (int)(Mc * lambda) : (int)ceil(X);
         const int base = prefix[cell];
         int count = 0;
         double smax = sigvrmax[cell];

@@ -374,6 +374,25 @@ void fill_f64(double *p, double v, long long n, hipStream_t stream, Profiler *pr
     DSMC_LAUNCH(prof, "fill", (double)n * 8.0, k_fill_f64, fill_grid(n), 256, stream, p, v, n);
 }
 
+// Streaming copy (16-byte vectors, grid-stride): the bandwidth reference used
+// to calibrate the HBM-traffic counters.
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4 *__restrict__ src,
+                                                     uint4 *__restrict__ dst, long long n16)
+{
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n16; i += (long long)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+void stream_copy(void *dst, const void *src, size_t bytes, hipStream_t stream)
+{
+    const long long n16 = (long long)(bytes / 16);
+    if (n16 <= 0) return;
+    const int grid = (int)std::min<long long>((n16 + 255) / 256, 256LL * 64);
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, stream, (const uint4 *)src,
+                       (uint4 *)dst, n16);
+    DSMC_HIP(hipGetLastError());
+}
+
 __global__ void k_philox_fill(uint32_t *out, int n, U32x4 ctr, uint32_t k0, uint32_t k1)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

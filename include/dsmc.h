@@ -42,6 +42,11 @@ typedef enum dsmc_status {
 
 enum { DSMC_TRAP_IP = 0, DSMC_TRAP_QUADRUPOLE = 1, DSMC_TRAP_BOX = 2 };
 enum { DSMC_FP64_PLAIN = 0, DSMC_FP32_KAHAN = 1 };
+/* DSMC_PAIR_REFERENCE reproduces collisions.cu:320-322 and :400-413 exactly:
+ * pairs floor(u (n-1)) (the last atom of a cell is never chosen) and the
+ * trial count (int)(Mc * (ceil(X) / Mc)), which rounds to ceil(X) - 1 for some
+ * Mc.  DSMC_PAIR_UNIFORM draws unordered pairs uniformly and runs ceil(X)
+ * trials. */
 enum { DSMC_PAIR_UNIFORM = 0, DSMC_PAIR_REFERENCE = 1 };
 enum { DSMC_EVAP_SPIN = 1, DSMC_EVAP_ENERGY = 2 };      /* bit flags          */
 
@@ -70,7 +75,7 @@ typedef struct dsmc_trap {
 
 typedef struct dsmc_config {
     int    precision;       /* DSMC_FP64_PLAIN or DSMC_FP32_KAHAN                 */
-    int    pair_mode;       /* DSMC_PAIR_*: collisions.cu:400-413 or unbiased     */
+    int    pair_mode;       /* DSMC_PAIR_*                                        */
     int    evaporation;     /* DSMC_EVAP_* flags, 0 = none                        */
     int    spin;            /* evolve spins (wavefunction.cu:11-73)               */
     int    spin_substeps;   /* spin rotations per motion step (>= 1)              */
@@ -249,6 +254,12 @@ dsmc_status dsmc_remove_if(int *d_values, const int *d_stencil, int n, int *n_ou
 /* Philox4x32-10 blocks out[4 i .. 4 i + 3] for counters (c0 + i, c1, c2, c3). */
 dsmc_status dsmc_philox(unsigned int *d_out, int n, const unsigned int ctr[4],
                         const unsigned int key[2], void *stream);
+
+/* Streaming device-to-device copy of `bytes` (a multiple of 16), launched
+ * `repeats` times on the default stream; *ms = mean time per copy.  No
+ * reference counterpart: the bandwidth reference against which the step's
+ * kernels and the HBM-traffic counters are calibrated. */
+dsmc_status dsmc_stream_copy(void *d_dst, const void *d_src, size_t bytes, int repeats, double *ms);
 
 /* Device memory helpers for hosts without their own allocator. */
 dsmc_status dsmc_device_alloc(void **d_ptr, size_t bytes);

@@ -299,8 +299,12 @@ long long orc_collide(const orc_params *p, orc_d3 *vel, double *sigvrmax,
         int n = prefix[cell + 1] - prefix[cell];
         if (n <= 1) continue;
         double Mc = 0.5 * (n - 1) * n;
-        double lambda = ceil(Mc * p->alpha * lpc * p->dt * sigvrmax[cell] / cellVolume) / Mc;
-        int Ncol = (int)(Mc * lambda);
+        double X = Mc * p->alpha * lpc * p->dt * sigvrmax[cell] / cellVolume;
+        double lambda = ceil(X) / Mc;
+        /* collisions.cu:322 truncates Mc * (ceil(X) / Mc), which is 1 ulp below
+         * ceil(X) for some Mc (n = 23, 34, 59, ...) and then drops a trial; the
+         * reference pair mode keeps that, the unbiased mode uses ceil(X). */
+        int Ncol = p->pair_mode == ORC_PAIR_REFERENCE ? (int)(Mc * lambda) : (int)ceil(X);
         int base = prefix[cell];
         collisionCount[cell] = 0;
         for (int t = 0; t < Ncol; ++t) {

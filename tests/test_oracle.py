@@ -1,0 +1,417 @@
+"""CPU pins of the oracle (oracle/dsmc_oracle.c): every function is checked
+against something other than a retyped formula -- published known answers,
+library routines, closed forms of the dynamics, conservation laws, and the
+reference's own published numbers (homogeneousGasCollisions.py:27-33,
+collisionChecker.py:34, energyChecker.py:185).
+"""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from cudadsmc_amd import configs
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+# ---------------------------------------------------------------------------
+# Philox4x32-10 and the draws
+# ---------------------------------------------------------------------------
+def test_philox_known_answers(orc):
+    kat = json.loads((GOLDEN / "philox_kat.json").read_text())
+    for case in kat["vectors"]:
+        assert orc.philox(case["ctr"], case["key"]) == case["out"]
+
+
+def test_u01_grid(orc):
+    # (k + 1/2) 2^-52 from the top 52 of 64 bits: both ends strictly inside (0,1)
+    assert orc.u01(0, 0) == 2.0 ** -53
+    assert orc.u01(0xFFFFFFFF, 0xFFFFFFFF) == 1.0 - 2.0 ** -53
+    assert orc.u01(0x80000000, 0) == 0.5 + 2.0 ** -53
+    assert orc.u01(0x7FFFFFFF, 0xFFFFF000) == 0.5 - 2.0 ** -53
+
+
+def test_uint_below(orc):
+    for n in (1, 2, 3, 7, 1000, 2 ** 31):
+        for r in (0, 1, 0x7FFFFFFF, 0xFFFFFFFF):
+            v = orc.uint_below(r, n)
+            assert 0 <= v < n
+            assert v == (r * n) >> 32
+
+
+# ---------------------------------------------------------------------------
+# Trap field and motion
+# ---------------------------------------------------------------------------
+def _accel(orc, p, r):
+    q = orc.Params.from_buffer_copy(p)
+    q.dt = 0.0
+    pos = np.array([r], dtype=np.float64)
+    vel = np.zeros((1, 3))
+    acc = np.zeros((1, 3))
+    orc.move_atoms(q, pos, vel, acc, np.array([0], np.int32))
+    return acc[0]
+
+
+def _potential(run, r):
+    t = run.traps[0]
+    x, y, z = r
+    if t.kind == "quadrupole":
+        return t.mu * t.dBdz * math.sqrt(x * x + y * y + 4 * z * z)
+    Bx = t.dBdx * x - 0.5 * t.d2Bdx2 * x * z
+    By = -t.dBdx * y - 0.5 * t.d2Bdx2 * y * z
+    Bz = t.B0 + 0.5 * t.d2Bdx2 * (z * z - 0.5 * (x * x + y * y))
+    return t.mu * (math.sqrt(Bx * Bx + By * By + Bz * Bz) - t.B0)
+
+
+@pytest.mark.parametrize("kind", ["ip", "quadrupole"])
+def test_force_is_minus_gradient_of_potential(orc, kind):
+    """a = -grad U / m by central differences of an independently written U
+    (IP field of moveAtoms.cu:135-142 and the quadrupole |B| = B' rho)."""
+    run = configs.reference() if kind == "ip" else configs.quadrupole_1e4()
+    p = orc.params_from(run)
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        r = rng.normal(0, 5e-5, 3)
+        a = _accel(orc, p, r)
+        g = np.zeros(3)
+        h = 1e-10
+        for d in range(3):
+            e = np.zeros(3); e[d] = h
+            g[d] = (_potential(run, r + e) - _potential(run, r - e)) / (2 * h)
+        np.testing.assert_allclose(a, -g / run.mass, rtol=2e-5, atol=1e-9 * np.abs(g).max() / run.mass)
+
+
+def test_verlet_exact_for_constant_force(orc):
+    """On the +x axis of the quadrupole the force is the constant -mu B'/m x^:
+    velocity Verlet reproduces x0 + v0 t - g t^2 / 2 to rounding."""
+    run = configs.quadrupole_1e4()
+    p = orc.params_from(run)
+    t = run.traps[0]
+    g = t.mu * t.dBdz / run.mass
+    n = 64
+    x0 = np.linspace(1e-4, 2e-4, n)
+    v0 = np.linspace(-0.01, 0.05, n)
+    pos = np.zeros((n, 3)); pos[:, 0] = x0
+    vel = np.zeros((n, 3)); vel[:, 0] = v0
+    acc = np.zeros((n, 3)); acc[:, 0] = -g
+    ids = np.arange(n, dtype=np.int32)
+    steps = 200
+    for _ in range(steps):
+        orc.move_atoms(p, pos, vel, acc, ids)
+    T = steps * run.dt
+    np.testing.assert_allclose(pos[:, 0], x0 + v0 * T - 0.5 * g * T * T, rtol=1e-12)
+    np.testing.assert_allclose(vel[:, 0], v0 - g * T, rtol=1e-11, atol=1e-15)
+    assert np.all(pos[:, 1:] == 0.0)
+
+
+def test_periodic_free_flight(orc):
+    run = configs.box_thermalise("fp64", n_atoms=1000, nc=8).with_(sigma=0.0)
+    p = orc.params_from(run)
+    hw = run.traps[0].half_width[0]
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-hw, hw, (1000, 3))
+    vel = rng.normal(0, 0.05, (1000, 3))
+    acc = np.zeros((1000, 3))
+    x0, v0 = pos.copy(), vel.copy()
+    ids = np.arange(1000, dtype=np.int32)
+    for _ in range(500):
+        orc.move_atoms(p, pos, vel, acc, ids)
+    expect = (x0 + v0 * 500 * run.dt + hw) % (2 * hw) - hw
+    np.testing.assert_allclose(pos, expect, rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(vel, v0)
+    assert np.all(pos >= -hw) and np.all(pos < hw)
+
+
+def _energy_series(orc, run, steps, every=10):
+    ref = orc.Reference(run)
+    E = []
+    for k in range(steps):
+        if k % every == 0:
+            e = orc.energy(ref.p, ref.s["pos"], ref.s["vel"], ref.s["atom_id"])
+            E.append(e[0] + e[1])
+        ref.motion_step()
+    return np.array(E)
+
+
+def test_energy_conservation_ip(orc):
+    """energyChecker.py:170,185: max |dE/E| below 1e-5 (its 1e-3 %) for the
+    reference's own trap and step without collisions."""
+    run = configs.reference(20, 3000).with_(sigma=0.0, spin=False, spin_loss=False)
+    E = _energy_series(orc, run, 2000)
+    assert np.max(np.abs(E - E[0]) / E[0]) < 1e-5
+
+
+def test_energy_error_second_order_in_dt(orc):
+    """energyComparison.py:60: log-log slope of max |dE| against dt is 2 for
+    velocity Verlet (1 for symplectic Euler)."""
+    base = configs.reference(20, 2000).with_(sigma=0.0, spin=False, spin_loss=False)
+    errs = []
+    dts = [4e-6, 2e-6, 1e-6]
+    total = 4e-3
+    for dt in dts:
+        E = _energy_series(orc, base.with_(dt=dt), int(round(total / dt)), every=1)
+        errs.append(np.max(np.abs(E - E[0])))
+    slope = np.polyfit(np.log(dts), np.log(errs), 1)[0]
+    assert 1.7 < slope < 2.3, slope
+
+
+# ---------------------------------------------------------------------------
+# Index, sort, cell table, compaction
+# ---------------------------------------------------------------------------
+def test_index_hand_placed(orc):
+    """Cell centres, lower faces, one ulp below a face, outside and NaN
+    (collisions.cu:187-219): ids built from the placing indices."""
+    run = configs.quadrupole_1e4()
+    p = orc.params_from(run)
+    nc = p.nc[0]
+    hw = p.half_width[0]
+    L = (2.0 * hw) / nc
+    rng = np.random.default_rng(5)
+    idx = rng.integers(0, nc, (500, 3))
+    centre = -hw + (idx + 0.5) * L
+    expect = idx[:, 2] * nc * nc + idx[:, 1] * nc + idx[:, 0]
+    np.testing.assert_array_equal(orc.index_atoms(p, centre, np.arange(500, dtype=np.int32)),
+                                  expect)
+    # exactly on the lower face of cell 0 along x, and beyond the grid
+    pts = np.array([[-hw, 0, 0], [hw, 0, 0], [np.nextafter(-hw, -1), 0, 0], [0, 0, 2 * hw],
+                    [np.nan, 0, 0], [0, np.inf, 0]])
+    cells = orc.index_atoms(p, pts, np.arange(len(pts), dtype=np.int32))
+    mid = nc // 2
+    assert cells[0] == mid * nc * nc + mid * nc + 0
+    assert all(c == nc ** 3 for c in cells[1:])
+
+
+def test_sort_matches_stable_argsort(orc):
+    rng = np.random.default_rng(7)
+    keys = rng.integers(0, 1001, 50000).astype(np.int32)
+    ids = rng.permutation(50000).astype(np.int32)
+    k, v = orc.sort_arrays(keys, ids, 1001)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(k, keys[order])
+    np.testing.assert_array_equal(v, ids[order])
+
+
+def test_cell_table_matches_bincount(orc):
+    rng = np.random.default_rng(8)
+    nc = 9 ** 3
+    keys = np.sort(rng.integers(0, nc + 1, 20000)).astype(np.int32)
+    se, count, prefix = orc.cell_table(keys, nc)
+    np.testing.assert_array_equal(count, np.bincount(keys, minlength=nc + 1))
+    np.testing.assert_array_equal(prefix, np.concatenate([[0], np.cumsum(count)[:-1]]))
+    occ = count > 0
+    assert np.all(se[~occ] == -1)
+    np.testing.assert_array_equal(se[occ, 1] - se[occ, 0] + 1, count[occ])
+    np.testing.assert_array_equal(keys[se[occ, 0]], np.nonzero(occ)[0])
+
+
+def test_remove_if_matches_mask(orc):
+    rng = np.random.default_rng(9)
+    ids = rng.permutation(10000).astype(np.int32)
+    tag = (rng.random(10000) < 0.37).astype(np.int32)
+    np.testing.assert_array_equal(orc.remove_if(ids, tag), ids[tag == 0])
+
+
+def test_d1_quirk_equals_intended_only_at_identity(orc):
+    """findAtomIndex writes cellID[atomID[slot]] (collisions.cu:181) while the
+    sort pairs cellID[slot] with atomID[slot]: identical to the intended
+    key[slot] = cell(atom in slot) only while atomID is the identity."""
+    run = configs.quadrupole_1e4()
+    p = orc.params_from(run)
+    st = orc.init_distribution(p, 5000, run.traps[0].T)
+    ident = np.arange(5000, dtype=np.int32)
+    a = orc.index_atoms(p, st["pos"], ident, d1_quirk=True)
+    b = orc.index_atoms(p, st["pos"], ident, d1_quirk=False)
+    np.testing.assert_array_equal(a, b)
+    perm = np.random.default_rng(2).permutation(5000).astype(np.int32)
+    a = orc.index_atoms(p, st["pos"], perm, d1_quirk=True)
+    b = orc.index_atoms(p, st["pos"], perm, d1_quirk=False)
+    assert not np.array_equal(a, b)
+    np.testing.assert_array_equal(np.sort(a), np.sort(b))
+
+
+# ---------------------------------------------------------------------------
+# Collisions
+# ---------------------------------------------------------------------------
+def _dense_collision_state(orc, n=20000, nc=6, n_real=1e10, seed=4, pair_mode="uniform"):
+    run = configs.Run(traps=[configs.quadrupole_trap(n, nc, n_real=n_real)], precision="fp64",
+                      dt=1e-6, seed=seed, pair_mode=pair_mode)
+    ref = orc.Reference(run)
+    return run, ref
+
+
+@pytest.mark.parametrize("pair_mode", ["uniform", "reference"])
+def test_collide_conserves_momentum_and_energy(orc, pair_mode):
+    run, ref = _dense_collision_state(orc, pair_mode=pair_mode)
+    v0 = ref.s["vel"].copy()
+    info = ref.collision_step()
+    v1 = ref.s["vel"]
+    assert info["total"] > 500
+    changed = np.any(v0 != v1, axis=1)
+    assert changed.sum() >= info["total"]
+    np.testing.assert_allclose(v1.sum(axis=0), v0.sum(axis=0), rtol=0,
+                               atol=1e-13 * np.abs(v0).sum())
+    e0, e1 = np.sum(v0 ** 2), np.sum(v1 ** 2)
+    assert abs(e1 - e0) <= 1e-13 * e0
+    # sigma v_r,max only grows
+    init = orc.sigvrmax_init(ref.p, run.traps[0].T)
+    assert np.all(ref.sigvrmax >= init)
+
+
+def test_scatter_directions_isotropic(orc):
+    """Marsaglia (1972) directions replace the three normalised Gaussians of
+    collisions.cu:537-546: mean 0 and second moments delta_ij / 3."""
+    hw = 1e-3
+    ncell = 30
+    n = 2 * ncell ** 3
+    run = configs.Run(traps=[configs.Trap(kind="box", n_atoms=n, nc=(ncell,) * 3,
+                                          half_width=(hw,) * 3, T=20e-6, mu=0.0, n_real=n * 2e9)],
+                      precision="fp64", dt=1e-6, seed=9)
+    p = orc.params_from(run)
+    L = 2 * hw / ncell
+    idx = np.repeat(np.arange(ncell ** 3), 2)
+    pos = np.stack([-hw + ((idx % ncell) + 0.5) * L, -hw + (((idx // ncell) % ncell) + 0.5) * L,
+                    -hw + ((idx // ncell ** 2) + 0.5) * L], axis=1)
+    vel = np.random.default_rng(1).normal(0, 0.05, (n, 3))
+    ref = orc.Reference(run, pos=pos, vel=vel)
+    v0 = ref.s["vel"].copy()
+    ref.collision_step()
+    v1 = ref.s["vel"]
+    a, b = v1[0::2], v1[1::2]
+    hit = np.any(v0[0::2] != a, axis=1)
+    assert hit.sum() > 5000
+    d = (b - a)[hit]
+    nh = d / np.linalg.norm(d, axis=1)[:, None]
+    m = len(nh)
+    assert np.all(np.abs(nh.mean(axis=0)) < 5 / math.sqrt(m))
+    second = nh.T @ nh / m
+    np.testing.assert_allclose(second, np.eye(3) / 3, atol=6 / math.sqrt(m))
+
+
+def test_homogeneous_collision_rate(orc):
+    """homogeneousGasCollisions.py:33: n sigma <v_r> / 2 = 43.5568 s^-1 for
+    N_real = 1e10 in (2 mm)^3 at 20 uK (collisionComparison.py:22-33); the
+    reference's own runs give 43.28 .. 43.66 s^-1 for 1e4 .. 1e6 test
+    particles (homogeneousGasCollisions.py:30-31)."""
+    gold = json.loads((GOLDEN / "homogeneous_rates.json").read_text())
+    analytic = configs.analytic_homogeneous_rate(1e10 / (2e-3) ** 3, 20e-6)
+    assert abs(analytic - gold["analytic"]) < 5e-5
+    n, ncell, steps, dt = 50000, 10, 3000, 5e-6
+    run = configs.Run(traps=[configs.Trap(kind="box", n_atoms=n, nc=(ncell,) * 3,
+                                          half_width=(1e-3,) * 3, T=20e-6, mu=0.0, n_real=1e10)],
+                      precision="fp64", dt=dt, seed=5)
+    ref = orc.Reference(run)
+    ref.step(steps)
+    rate = ref.collisions / (n * steps * dt)
+    expected_err = 1.0 / math.sqrt(ref.collisions)
+    assert abs(rate / analytic - 1) < 4 * expected_err + 0.005, (rate, analytic)
+    lo = min(gold["computed"]["10"][2:] + gold["computed"]["5"][2:])
+    hi = max(gold["computed"]["10"][2:] + gold["computed"]["5"][2:])
+    assert lo * (1 - 4 * expected_err) < rate < hi * (1 + 4 * expected_err)
+
+
+def test_reference_trial_count_truncation(orc):
+    """collisions.cu:320-322 computes the trial count as (int)(Mc * (ceil(X) /
+    Mc)); for pair counts Mc = n (n - 1) / 2 with n = 23, 34, 59, ... the
+    product rounds below ceil(X) and a trial is lost.  In a homogeneous gas
+    with about 50 atoms per cell that removes about 3.5 % of the collisions:
+    the reference pair mode reproduces it, the unbiased mode does not."""
+    bad = [n for n in range(2, 200) if int((0.5 * (n - 1) * n) * (1.0 / (0.5 * (n - 1) * n))) == 0]
+    assert bad[:5] == [23, 34, 59, 76, 84]
+    rates = {}
+    for mode in ("uniform", "reference"):
+        n, steps, dt = 50000, 1500, 5e-6
+        run = configs.Run(traps=[configs.Trap(kind="box", n_atoms=n, nc=(10,) * 3,
+                                              half_width=(1e-3,) * 3, T=20e-6, mu=0.0,
+                                              n_real=1e10)],
+                          precision="fp64", dt=dt, seed=6, pair_mode=mode)
+        ref = orc.Reference(run)
+        ref.step(steps)
+        rates[mode] = ref.collisions / (n * steps * dt)
+    assert rates["reference"] < rates["uniform"] * 0.985
+
+
+def test_trap_rate_formula_matches_checker():
+    """collisionChecker.py:34 at its own parameters (N = 1e6, 10 uK, B' = 2.5,
+    gs = 0.5): 18.135 s^-1; our formula takes the moment explicitly."""
+    r = configs.analytic_trap_rate(1e6, 10e-6, 2.5, mu=0.5 * configs.GS * configs.MUB)
+    assert abs(r - 18.135) < 5e-3
+
+
+# ---------------------------------------------------------------------------
+# Spin
+# ---------------------------------------------------------------------------
+def test_spinor_rotation_equals_bloch_rotation(orc):
+    """The SU(2) update of wavefunction.cu:47-70 maps to a rotation of the
+    Bloch vector by phi = 2 theta about B (the form the FP32 path evolves)."""
+    run = configs.reference(20, 500)
+    p = orc.params_from(run)
+    st = orc.init_distribution(p, 500, 20e-6, spin=True)
+    rng = np.random.default_rng(3)
+    u = rng.normal(size=(500, 2)); d = rng.normal(size=(500, 2))
+    nrm = np.sqrt((u ** 2).sum(1) + (d ** 2).sum(1))[:, None]
+    u /= nrm; d /= nrm
+    def bloch(u, d):
+        return np.stack([2 * (u[:, 0] * d[:, 0] + u[:, 1] * d[:, 1]),
+                         2 * (u[:, 0] * d[:, 1] - u[:, 1] * d[:, 0]),
+                         (u ** 2).sum(1) - (d ** 2).sum(1)], axis=1)
+    s = np.ascontiguousarray(bloch(u, d))
+    ids = np.arange(500, dtype=np.int32)
+    dt = 3e-7
+    uu, dd = np.ascontiguousarray(u), np.ascontiguousarray(d)
+    orc.evolve_wavefunction(p, st["pos"], uu, dd, ids, dt)
+    orc.rotate_bloch(p, st["pos"], s, ids, dt)
+    np.testing.assert_allclose(bloch(uu, dd), s, atol=1e-12)
+
+
+def test_aligned_spin_and_uniform_precession(orc):
+    """getAlignedSpinUp/Dn (setUp.cu:236-252) give a Bloch vector along B; a
+    spin perpendicular to a uniform field precesses by exactly
+    phi = gs muB |B| t / hbar."""
+    run = configs.reference(20, 200)
+    p = orc.params_from(run)
+    st = orc.init_distribution(p, 200, 20e-6, spin=True)
+    u, d = st["psi_up"], st["psi_dn"]
+    s = np.stack([2 * (u[:, 0] * d[:, 0] + u[:, 1] * d[:, 1]),
+                  2 * (u[:, 0] * d[:, 1] - u[:, 1] * d[:, 0]),
+                  (u ** 2).sum(1) - (d ** 2).sum(1)], axis=1)
+    # B at the trap centre is B0 z^ (IP field), so compare at the origin
+    pos0 = np.zeros((1, 3))
+    sx = np.array([[1.0, 0.0, 0.0]])
+    t = 7e-8
+    orc.rotate_bloch(p, pos0, sx, np.array([0], np.int32), t)
+    phi = run.gs * configs.MUB * 0.01 * t / configs.HBAR
+    np.testing.assert_allclose(sx[0], [math.cos(phi), math.sin(phi), 0.0], atol=1e-14)
+    # the aligned spinors point along B written out independently
+    for i in range(0, 200, 17):
+        x, y, z = st["pos"][i]
+        B = np.array([20 * x - 2e4 * x * z, -20 * y - 2e4 * y * z,
+                      0.01 + 2e4 * (z * z - 0.5 * (x * x + y * y))])
+        np.testing.assert_allclose(s[i], B / np.linalg.norm(B), atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Initial distribution
+# ---------------------------------------------------------------------------
+def test_quadrupole_initial_distribution_moments(orc):
+    """Positions: in (x, y, 2z) the radius is Gamma(3, r_th) -> <mu B' rho> = 3 kT;
+    velocities Maxwell: <m v^2 / 2> = 3 kT / 2."""
+    from scipy import stats
+    run = configs.quadrupole_1e4()
+    t = run.traps[0]
+    p = orc.params_from(run)
+    n = 40000
+    st = orc.init_distribution(p, n, t.T)
+    x, y, z = st["pos"].T
+    rho = np.sqrt(x * x + y * y + 4 * z * z)
+    kT = configs.KB * t.T
+    rth = kT / (t.mu * t.dBdz)
+    assert stats.kstest(rho / rth, stats.gamma(3).cdf).pvalue > 1e-3
+    ek = 0.5 * run.mass * (st["vel"] ** 2).sum(1)
+    assert abs(ek.mean() / (1.5 * kT) - 1) < 0.02
+    for d in range(3):
+        assert stats.kstest(st["vel"][:, d] / math.sqrt(kT / run.mass), "norm").pvalue > 1e-3
+    # isotropy of the direction in (x, y, 2z)
+    u = np.stack([x, y, 2 * z], 1) / rho[:, None]
+    assert np.all(np.abs(u.mean(0)) < 0.02)
