@@ -203,6 +203,7 @@ static TrapDev make_trapdev(const dsmc_config &c, const dsmc_trap &t, int index,
         d.hw[k] = t.half_width[k];
         d.gmin[k] = -1.0 * t.half_width[k];
         d.len[k] = (2.0 * t.half_width[k]) / (double)t.nc[k];
+        d.inv_len[k] = 1.0 / d.len[k];
         d.f_hw[k] = (float)t.half_width[k];
     }
     d.n_cells = t.nc[0] * t.nc[1] * t.nc[2];
@@ -565,7 +566,8 @@ static void fused_move(dsmc_ctx *x)
                       x->have_sort ? x->perm : nullptr, x->trap_start, n, x->key_a,
                       x->evap_pos_f, x->evap_vel_f, x->rec, x->step_dev, x->n_cap, x->stream,
                       &x->prof);
-    launch_obs_finalize(x->T, x->max_trap_atoms, x->block, x->trap_start, x->rec, x->coll_accum,
+    launch_obs_finalize(x->T, x->max_trap_atoms, x->block, x->cfg.mass, x->trap_start, x->rec,
+                        x->coll_accum,
                         x->done, x->step_dev, x->obs_log, x->obs_cap, x->stream, &x->prof);
     if (!x->have_sort)  // keep the count with the state it describes
         DSMC_HIP(hipMemcpyAsync(x->n_dev + (x->cur ^ 1), x->n_dev + x->cur, sizeof(int),

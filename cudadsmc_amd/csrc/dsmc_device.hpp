@@ -36,6 +36,7 @@ struct TrapDev {
     double hw[3];          // grid half width
     double gmin[3];        // -1.0 * hw          (collisions.cu:221-226)
     double len[3];         // (2.0 * hw) / nc    (collisions.cu:187-192)
+    double inv_len[3];     // 1.0 / len, for the fast index path
     double cell_vol;       // len.x * len.y * len.z (collisions.cu:319)
     double B0, dBdx, d2Bdx2, dBdz;
     double mu;             // J/T
@@ -223,6 +224,27 @@ __device__ __forceinline__ int cell_of(const TrapDev &t, double x, double y, dou
     return idx[2] * t.nc[0] * t.nc[1] + idx[1] * t.nc[0] + idx[0];
 }
 
+// Same result as cell_of, cheaper: the quotient is formed with the reciprocal
+// (|q' - q| <= 2 ulp(q)), and only when q' lies within 1e-9 of an integer -- far
+// more than 2 ulp for any grid below 2^20 cells a side -- is the exact
+// division of the reference formula evaluated.  NaN and infinities fail the
+// range test in both branches.
+__device__ __forceinline__ int cell_of_fast(const TrapDev &t, double x, double y, double z)
+{
+    const double p[3] = { x, y, z };
+    int idx[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double num = p[d] - t.gmin[d];
+        double q = floor(num * t.inv_len[d]);
+        const double frac = num * t.inv_len[d] - q;
+        if (frac < 1e-9 || frac > 1.0 - 1e-9) q = floor(num / t.len[d]);
+        if (!(q > -1.0 && q < (double)t.nc[d])) return t.n_cells;
+        idx[d] = (int)q;
+    }
+    return idx[2] * t.nc[0] * t.nc[1] + idx[1] * t.nc[0] + idx[0];
+}
+
 __device__ __forceinline__ double wrap_periodic(double x, double hw)
 {
     if (x >= hw) x -= 2.0 * hw;
@@ -245,22 +267,29 @@ __device__ __forceinline__ F3 ip_field_f(const TrapDev &t, float x, float y, flo
     return B;
 }
 
+// The FP32 form normalises with the hardware reciprocal square root (about
+// 1 ulp): the force feeds a second-order integrator whose step error is many
+// orders larger.
+__device__ __forceinline__ float rsqrt_f(float v) { return __builtin_amdgcn_rsqf(v); }
+
 __device__ __forceinline__ float field_dir_f(const TrapDev &t, float x, float y, float z, F3 &Bn)
 {
     if (t.kind == kTrapIP) {
         F3 B = ip_field_f(t, x, y, z);
-        float len = sqrtf(B.x * B.x + B.y * B.y + B.z * B.z);
-        float inv = 1.0f / len;
+        const float b2 = B.x * B.x + B.y * B.y + B.z * B.z;
+        const float inv = rsqrt_f(b2);
         Bn.x = B.x * inv; Bn.y = B.y * inv; Bn.z = B.z * inv;
-        return len;
+        return b2 * inv;
     }
     if (t.kind == kTrapQuad) {
-        float rho = sqrtf(x * x + y * y + 4.0f * (z * z));
-        if (rho > 0.0f) {
-            float inv = 1.0f / rho;
+        const float r2 = x * x + y * y + 4.0f * (z * z);
+        if (r2 > 0.0f) {
+            const float inv = rsqrt_f(r2);
             Bn.x = x * inv; Bn.y = y * inv; Bn.z = (-2.0f * z) * inv;
-        } else { Bn.x = 0.0f; Bn.y = 0.0f; Bn.z = 1.0f; }
-        return t.f_dBdz * rho;
+            return t.f_dBdz * (r2 * inv);
+        }
+        Bn.x = 0.0f; Bn.y = 0.0f; Bn.z = 1.0f;
+        return 0.0f;
     }
     Bn.x = 0.0f; Bn.y = 0.0f; Bn.z = 1.0f;
     return 0.0f;
@@ -271,7 +300,7 @@ __device__ __forceinline__ F3 accel_f(const TrapDev &t, float x, float y, float 
     F3 a = { 0.0f, 0.0f, 0.0f };
     if (t.kind == kTrapIP) {
         F3 B = ip_field_f(t, x, y, z);
-        float inv = 1.0f / sqrtf(B.x * B.x + B.y * B.y + B.z * B.z);
+        const float inv = rsqrt_f(B.x * B.x + B.y * B.y + B.z * B.z);
         F3 Bn = { B.x * inv, B.y * inv, B.z * inv };
         const float h = 0.5f * t.f_d2Bdx2;
         float gx = (t.f_dBdx - h * z) * Bn.x - h * x * Bn.z;
@@ -280,9 +309,9 @@ __device__ __forceinline__ F3 accel_f(const TrapDev &t, float x, float y, float 
         float p = t.f_neg_mu_over_m;
         a.x = p * gx; a.y = p * gy; a.z = p * gz;
     } else if (t.kind == kTrapQuad) {
-        float rho = sqrtf(x * x + y * y + 4.0f * (z * z));
-        if (rho > 0.0f) {
-            float g = (t.f_neg_mu_over_m * t.f_dBdz) / rho;
+        const float r2 = x * x + y * y + 4.0f * (z * z);
+        if (r2 > 0.0f) {
+            const float g = t.f_neg_mu_over_m * t.f_dBdz * rsqrt_f(r2);
             a.x = g * x; a.y = g * y; a.z = g * (4.0f * z);
         }
     }

@@ -21,6 +21,8 @@
 // so the loop can be captured once and replayed as a HIP graph.
 #include "fused.hpp"
 
+#include <cstdlib>
+
 namespace dsmc {
 
 constexpr int kMaxTrapsLds = 8192;
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(kFBlock) void k_index_fused(const TrapDev *__restri
     const double X = (double)s.p[0][i] + (double)s.cp[0][i];
     const double Y = (double)s.p[1][i] + (double)s.cp[1][i];
     const double Z = (double)s.p[2][i] + (double)s.cp[2][i];
-    key[i] = (uint32_t)(tr.key_base + cell_of(tr, X, Y, Z));
+    key[i] = (uint32_t)(tr.key_base + cell_of_fast(tr, X, Y, Z));
 }
 
 // ---------------------------------------------------------------------------
@@ -117,6 +119,8 @@ __global__ __launch_bounds__(kFBlock) void k_index_fused(const TrapDev *__restri
 // every trap, and the alive count (= first dead slot).  n_in + 1 threads: the
 // extra one closes the last trap ranges.
 // ---------------------------------------------------------------------------
+constexpr int kTableItems = 16;  // slots per thread of the trap-boundary scan
+
 __global__ __launch_bounds__(kFBlock) void k_table_fused(const uint32_t *__restrict__ key,
                                                          const int *__restrict__ key_base,
                                                          TableParams tp,
@@ -129,25 +133,31 @@ __global__ __launch_bounds__(kFBlock) void k_table_fused(const uint32_t *__restr
     const int T = tp.n_traps;
     load_table(kb, key_base, T + 1);
     const int n = *n_in_dev;
-    const int b0 = blockIdx.x * kFBlock;
+    const int b0 = blockIdx.x * (kFBlock * kTableItems);
     if (b0 > n) return;
-    // Fast path: a block whose keys all lie in one trap holds no trap boundary
-    // except possibly at its first slot.
+    const int b1 = min(n + 1, b0 + kFBlock * kTableItems);  // slots b0..b1-1, slot n closes
+    // A range whose keys all lie in one trap holds no trap boundary except
+    // possibly at its first slot: then only that slot (and slot n) is examined.
     if (threadIdx.x == 0) {
-        const int last = min(n, b0 + kFBlock) - 1;
+        const int last = min(n, b1) - 1;
         blk_traps[0] = b0 < n ? last_le(kb, T + 1, (int)key[b0]) : T;
         blk_traps[1] = last >= b0 ? last_le(kb, T + 1, (int)key[last]) : T;
     }
     __syncthreads();
-    const int i = b0 + threadIdx.x;
-    if (i > n) return;
-    if (i != b0 && i != n && blk_traps[0] == blk_traps[1]) return;
-    const uint32_t k = (i < n) ? key[i] : tp.dead_key;
-    const int cur = (i == n) ? T : last_le(kb, T + 1, (int)k);
-    const int prev = (i == 0) ? -1 : last_le(kb, T + 1, (int)key[i - 1]);
-    if (cur != prev) {
-        for (int t = prev + 1; t <= cur; ++t) trap_start[t] = i;
-        if (prev < T && cur == T) *n_out_dev = i;
+    const bool one_trap = blk_traps[0] == blk_traps[1];
+    for (int it = 0; it < kTableItems; ++it) {
+        const int i = b0 + it * kFBlock + threadIdx.x;
+        if (i >= b1) break;
+        if (one_trap && i != b0 && i != n) continue;
+        const uint32_t k = (i < n) ? key[i] : tp.dead_key;
+        const uint32_t kp = (i > 0) ? key[i - 1] : 0xFFFFFFFFu;
+        if (i > 0 && i < n && kp == k) continue;
+        const int cur = (i == n) ? T : last_le(kb, T + 1, (int)k);
+        const int prev = (i == 0) ? -1 : last_le(kb, T + 1, (int)kp);
+        if (cur != prev) {
+            for (int t = prev + 1; t <= cur; ++t) trap_start[t] = i;
+            if (prev < T && cur == T) *n_out_dev = i;
+        }
     }
 }
 
@@ -175,99 +185,194 @@ __global__ __launch_bounds__(kFBlock) void k_cell_range_fused(const uint32_t *__
 // from value + remainder and stores the result back as float + remainder, so
 // momentum and energy are conserved to FP64 rounding.
 // ---------------------------------------------------------------------------
-// Trials of one cell whose first sorted slot is i (collisions.cu:306-441).
-__device__ __forceinline__ void collide_cell(const TrapDev *__restrict__ traps, const int *kb, int T,
-                                             const CollideParams &cp,
-                                             const uint32_t *__restrict__ key,
-                                             const uint32_t *__restrict__ perm,
-                                             const FusedPtrs &s, double *__restrict__ sigvrmax,
-                                             unsigned long long *__restrict__ coll_accum,
-                                             const long long *__restrict__ step_dev, int i, int end)
+// Cell state for the trials of one cell (collisions.cu:306-322).
+struct CellJob {
+    int      i;       // first sorted slot
+    int      n;       // atoms in the cell
+    int      t;       // trap
+    uint32_t cell;    // local cell (Philox counter word 0)
+    uint32_t k;       // composite key (sigma v_r,max index)
+    int      ncol;    // trials
+    float    pc;      // acceptance probability per unit |v_r|
+    double   smax;    // sigma v_r,max at entry
+};
+
+__device__ __forceinline__ bool cell_setup(const TrapDev *__restrict__ traps, const int *kb, int T,
+                                           const CollideParams &cp,
+                                           const uint32_t *__restrict__ key,
+                                           const double *__restrict__ sigvrmax, int i, int end,
+                                           CellJob &c)
 {
-    const int n = end - i;
-    if (n < 2) return;
-    const uint32_t k = key[i];
-    const int t = last_le(kb, T + 1, (int)k);
-    const TrapDev &tr = traps[t];
-    const uint32_t cell = k - (uint32_t)tr.key_base;
-    if ((int)cell >= tr.n_cells) return;  // overflow cell: no collisions
-    const uint32_t step = (uint32_t)*step_dev;
+    c.i = i;
+    c.n = end - i;
+    c.ncol = 0;
+    if (c.n < 2) return false;
+    c.k = key[i];
+    c.t = last_le(kb, T + 1, (int)c.k);
+    const TrapDev &tr = traps[c.t];
+    c.cell = c.k - (uint32_t)tr.key_base;
+    if ((int)c.cell >= tr.n_cells) return false;  // overflow cell: no collisions
     const double lpc = cp.lpc;
-    double smax = sigvrmax[k];
-    const double Mc = 0.5 * (n - 1) * n;
-    const double X = Mc * tr.alpha * lpc * cp.dt * smax / tr.cell_vol;
+    c.smax = sigvrmax[c.k];
+    const double Mc = 0.5 * (c.n - 1) * c.n;
+    const double X = Mc * tr.alpha * lpc * cp.dt * c.smax / tr.cell_vol;
     const double lambda = ceil(X) / Mc;
     // reference pair mode keeps collisions.cu:322's truncation of Mc * lambda
-    const int Ncol = cp.pair_mode ? (int)(Mc * lambda) : (int)ceil(X);
-    const float pc = (float)(tr.alpha * lpc * cp.dt / tr.cell_vol * cp.sigma / lambda);
-    int count = 0;
-    for (int tr_i = 0; tr_i < Ncol; ++tr_i) {
-        int ia = 0, ib = 1;
-        float u_acc;
-        if (cp.pair_mode == 0) {
-            const U32x4 o = draw(cp.seed, tr.stream, cell, step, (uint32_t)tr_i, kPurposeCollide, 0);
-            ia = (int)uint_below(o.x, (uint32_t)n);
-            ib = (int)uint_below(o.y, (uint32_t)(n - 1));
-            ib += (ib >= ia);
-            u_acc = u01f(o.z);
-        } else {
-            if (n > 2) {
-                ia = 0; ib = 0;
-                for (uint32_t q = 0; q < 64 && ia == ib; ++q) {
-                    const U32x4 o = draw(cp.seed, tr.stream, cell, step, (uint32_t)tr_i, kPurposeCollide, 16 + q);
-                    ia = (int)floor(u01(o.x, o.y) * (n - 1));
-                    ib = (int)floor(u01(o.z, o.w) * (n - 1));
-                }
-                if (ia == ib) { ia = 0; ib = 1; }
+    c.ncol = cp.pair_mode ? (int)(Mc * lambda) : (int)ceil(X);
+    c.pc = (float)(tr.alpha * lpc * cp.dt / tr.cell_vol * cp.sigma / lambda);
+    return c.ncol > 0;
+}
+
+// Pair (slot offsets within the cell) and acceptance uniform of trial tr
+// (collisions.cu:400-425).
+__device__ __forceinline__ void trial_draw(const CollideParams &cp, uint32_t stream, uint32_t cell,
+                                           uint32_t step, int tr, int n, int &ia, int &ib,
+                                           float &u_acc)
+{
+    ia = 0; ib = 1;
+    if (cp.pair_mode == 0) {
+        const U32x4 o = draw(cp.seed, stream, cell, step, (uint32_t)tr, kPurposeCollide, 0);
+        ia = (int)uint_below(o.x, (uint32_t)n);
+        ib = (int)uint_below(o.y, (uint32_t)(n - 1));
+        ib += (ib >= ia);
+        u_acc = u01f(o.z);
+    } else {
+        if (n > 2) {
+            ia = 0; ib = 0;
+            for (uint32_t q = 0; q < 64 && ia == ib; ++q) {
+                const U32x4 o = draw(cp.seed, stream, cell, step, (uint32_t)tr, kPurposeCollide, 16 + q);
+                ia = (int)floor(u01(o.x, o.y) * (n - 1));
+                ib = (int)floor(u01(o.z, o.w) * (n - 1));
             }
-            const U32x4 o = draw(cp.seed, tr.stream, cell, step, (uint32_t)tr_i, kPurposeCollide, 1);
-            u_acc = u01f(o.x);
+            if (ia == ib) { ia = 0; ib = 1; }
         }
-        const uint32_t a = perm[i + ia], b = perm[i + ib];
-        const float dx = s.v[0][a] - s.v[0][b];
-        const float dy = s.v[1][a] - s.v[1][b];
-        const float dz = s.v[2][a] - s.v[2][b];
-        const float vr = sqrtf(dx * dx + dy * dy + dz * dz);
+        const U32x4 o = draw(cp.seed, stream, cell, step, (uint32_t)tr, kPurposeCollide, 1);
+        u_acc = u01f(o.x);
+    }
+}
+
+__device__ __forceinline__ float rel_speed(const FusedPtrs &s, uint32_t a, uint32_t b)
+{
+    const float dx = s.v[0][a] - s.v[0][b];
+    const float dy = s.v[1][a] - s.v[1][b];
+    const float dz = s.v[2][a] - s.v[2][b];
+    return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+// Accepted collision of atoms a and b of trial tr, in FP64 from value +
+// remainder, stored back as float + remainder (collisions.cu:426-436).
+__device__ __forceinline__ void scatter_pair(const CollideParams &cp, uint32_t stream, uint32_t cell,
+                                             uint32_t step, int tr, const FusedPtrs &s, uint32_t a,
+                                             uint32_t b)
+{
+    const double vax = (double)s.v[0][a] + (double)s.cv[0][a];
+    const double vay = (double)s.v[1][a] + (double)s.cv[1][a];
+    const double vaz = (double)s.v[2][a] + (double)s.cv[2][a];
+    const double vbx = (double)s.v[0][b] + (double)s.cv[0][b];
+    const double vby = (double)s.v[1][b] + (double)s.cv[1][b];
+    const double vbz = (double)s.v[2][b] + (double)s.cv[2][b];
+    const double ddx = vax - vbx, ddy = vay - vby, ddz = vaz - vbz;
+    const double magVrel = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+    const double cx = 0.5 * (vax + vbx), cy = 0.5 * (vay + vby), cz = 0.5 * (vaz + vbz);
+    double nx, ny, nz;
+    sphere_point(cp.seed, stream, cell, step, (uint32_t)tr, kPurposeCollide, 2, nx, ny, nz);
+    const double wx = magVrel * nx, wy = magVrel * ny, wz = magVrel * nz;
+    split_df(cx - 0.5 * wx, s.v[0][a], s.cv[0][a]);
+    split_df(cy - 0.5 * wy, s.v[1][a], s.cv[1][a]);
+    split_df(cz - 0.5 * wz, s.v[2][a], s.cv[2][a]);
+    split_df(cx + 0.5 * wx, s.v[0][b], s.cv[0][b]);
+    split_df(cy + 0.5 * wy, s.v[1][b], s.cv[1][b]);
+    split_df(cz + 0.5 * wz, s.v[2][b], s.cv[2][b]);
+}
+
+// The trials of one cell in sequence, by one lane (collisions.cu:397-441).
+__device__ __forceinline__ int cell_trials_serial(const CollideParams &cp, const CellJob &c,
+                                                  uint32_t stream, uint32_t step,
+                                                  const uint32_t *__restrict__ perm,
+                                                  const FusedPtrs &s, double &smax)
+{
+    int count = 0;
+    for (int tr = 0; tr < c.ncol; ++tr) {
+        int ia, ib;
+        float u;
+        trial_draw(cp, stream, c.cell, step, tr, c.n, ia, ib, u);
+        const uint32_t a = perm[c.i + ia], b = perm[c.i + ib];
+        const float vr = rel_speed(s, a, b);
         const double sv = (double)vr * cp.sigma;
         if (sv > smax) smax = sv;
-        if (pc * vr > u_acc) {
-            const double vax = (double)s.v[0][a] + (double)s.cv[0][a];
-            const double vay = (double)s.v[1][a] + (double)s.cv[1][a];
-            const double vaz = (double)s.v[2][a] + (double)s.cv[2][a];
-            const double vbx = (double)s.v[0][b] + (double)s.cv[0][b];
-            const double vby = (double)s.v[1][b] + (double)s.cv[1][b];
-            const double vbz = (double)s.v[2][b] + (double)s.cv[2][b];
-            const double ddx = vax - vbx, ddy = vay - vby, ddz = vaz - vbz;
-            const double magVrel = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-            const double cx = 0.5 * (vax + vbx), cy = 0.5 * (vay + vby), cz = 0.5 * (vaz + vbz);
-            double nx, ny, nz;
-            sphere_point(cp.seed, tr.stream, cell, step, (uint32_t)tr_i, kPurposeCollide, 2, nx, ny, nz);
-            const double wx = magVrel * nx, wy = magVrel * ny, wz = magVrel * nz;
-            split_df(cx - 0.5 * wx, s.v[0][a], s.cv[0][a]);
-            split_df(cy - 0.5 * wy, s.v[1][a], s.cv[1][a]);
-            split_df(cz - 0.5 * wz, s.v[2][a], s.cv[2][a]);
-            split_df(cx + 0.5 * wx, s.v[0][b], s.cv[0][b]);
-            split_df(cy + 0.5 * wy, s.v[1][b], s.cv[1][b]);
-            split_df(cz + 0.5 * wz, s.v[2][b], s.cv[2][b]);
+        if (c.pc * vr > u) {
+            scatter_pair(cp, stream, c.cell, step, tr, s, a, b);
             ++count;
         }
     }
-    sigvrmax[k] = smax;
-    if (count) atomicAdd(&coll_accum[t], (unsigned long long)count);
+    return count;
+}
+
+// The trials of one cell by the whole wave, with the result of the sequential
+// loop: 64 consecutive trials are evaluated at once; up to the first accepted
+// one they are exactly the sequential trials (no velocity changed before
+// them); that collision is applied and evaluation resumes after it.  All 64
+// lanes call this with the same job.
+__device__ __forceinline__ int cell_trials_wave(const CollideParams &cp, const CellJob &c,
+                                                uint32_t stream, uint32_t step,
+                                                const uint32_t *__restrict__ perm,
+                                                const FusedPtrs &s, double &smax)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    int count = 0;
+    int tr0 = 0;
+    while (tr0 < c.ncol) {
+        const int tr = tr0 + lane;
+        const bool active = tr < c.ncol;
+        uint32_t a = 0, b = 0;
+        double sv = 0.0;
+        bool acc = false;
+        if (active) {
+            int ia, ib;
+            float u;
+            trial_draw(cp, stream, c.cell, step, tr, c.n, ia, ib, u);
+            a = perm[c.i + ia];
+            b = perm[c.i + ib];
+            const float vr = rel_speed(s, a, b);
+            sv = (double)vr * cp.sigma;
+            acc = c.pc * vr > u;
+        }
+        const uint64_t mask = __ballot(acc);
+        const int f = mask ? __ffsll((unsigned long long)mask) - 1 : kWave;
+        double m = (active && lane <= f) ? sv : 0.0;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, kWave));
+        if (m > smax) smax = m;
+        if (f < kWave) {
+            if (lane == f) scatter_pair(cp, stream, c.cell, step, tr, s, a, b);
+            __threadfence();  // the next round reads the updated velocities
+            ++count;
+            tr0 += f + 1;
+        } else {
+            tr0 += kWave;
+        }
+    }
+    return count;
 }
 
 constexpr int kCollItems = 4;  // slots per lane scanned for cell heads
+constexpr int kWaveTrialsDefault = 16;
 
+// Cells with more than `wave_trials` trials are run by the whole wave after
+// the lanes have run the lighter cells one per lane.
 __global__ __launch_bounds__(kFBlock) void k_collide_fused(
     const TrapDev *__restrict__ traps, const int *__restrict__ key_base, int T, CollideParams cp,
     const uint32_t *__restrict__ key, const uint32_t *__restrict__ perm,
     const int *__restrict__ n_dev, FusedPtrs s, double *__restrict__ sigvrmax,
-    unsigned long long *__restrict__ coll_accum, const long long *__restrict__ step_dev)
+    unsigned long long *__restrict__ coll_accum, const long long *__restrict__ step_dev,
+    int wave_trials)
 {
     extern __shared__ int kb[];
     __shared__ int heads[kFBlock / kWave][kWave * kCollItems + 1];
+    __shared__ int heavy[kFBlock / kWave][kWave * kCollItems];
     load_table(kb, key_base, T + 1);
     const int n = *n_dev;
+    const uint32_t step = (uint32_t)*step_dev;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int chunk0 = (blockIdx.x * (kFBlock / kWave) + w) * (kWave * kCollItems);
     const int chunk1 = min(n, chunk0 + kWave * kCollItems);
@@ -286,9 +391,9 @@ __global__ __launch_bounds__(kFBlock) void k_collide_fused(
     if (lane == 0 && nh > 0) {
         // end of the last cell of the chunk: gallop forward through the keys
         const uint32_t k = key[heads[w][nh - 1]];
-        int lo = chunk1, step = 1;
-        while (lo + step - 1 < n && key[lo + step - 1] == k) { lo += step; step <<= 1; }
-        int hi = min(n, lo + step - 1);  // first slot with another key is in [lo, hi]
+        int lo = chunk1, stp = 1;
+        while (lo + stp - 1 < n && key[lo + stp - 1] == k) { lo += stp; stp <<= 1; }
+        int hi = min(n, lo + stp - 1);  // first slot with another key is in [lo, hi]
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (key[mid] == k) lo = mid + 1; else hi = mid;
@@ -296,9 +401,37 @@ __global__ __launch_bounds__(kFBlock) void k_collide_fused(
         heads[w][nh] = lo;
     }
     __syncthreads();
-    for (int h = lane; h < nh; h += kWave)
-        collide_cell(traps, kb, T, cp, key, perm, s, sigvrmax, coll_accum, step_dev, heads[w][h],
-                     heads[w][h + 1]);
+    int nheavy = 0;
+    for (int h0 = 0; h0 < nh; h0 += kWave) {
+        const int h = h0 + lane;
+        CellJob c;
+        const bool ok = h < nh && cell_setup(traps, kb, T, cp, key, sigvrmax, heads[w][h],
+                                             heads[w][h + 1], c);
+        const bool is_heavy = ok && c.ncol > wave_trials;
+        const uint64_t hb = __ballot(is_heavy);
+        if (is_heavy) heavy[w][nheavy + __popcll(hb & lanemask_lt())] = h;
+        nheavy += __popcll(hb);
+        if (ok && !is_heavy) {
+            const uint32_t stream = traps[c.t].stream;
+            double smax = c.smax;
+            const int cnt = cell_trials_serial(cp, c, stream, step, perm, s, smax);
+            sigvrmax[c.k] = smax;
+            if (cnt) atomicAdd(&coll_accum[c.t], (unsigned long long)cnt);
+        }
+    }
+    __syncthreads();
+    for (int q = 0; q < nheavy; ++q) {
+        const int h = heavy[w][q];
+        CellJob c;
+        (void)cell_setup(traps, kb, T, cp, key, sigvrmax, heads[w][h], heads[w][h + 1], c);
+        const uint32_t stream = traps[c.t].stream;
+        double smax = c.smax;
+        const int cnt = cell_trials_wave(cp, c, stream, step, perm, s, smax);
+        if (lane == 0) {
+            sigvrmax[c.k] = smax;
+            if (cnt) atomicAdd(&coll_accum[c.t], (unsigned long long)cnt);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -318,7 +451,7 @@ __global__ __launch_bounds__(BS) void k_move_fused(
     int rec_stride, double *__restrict__ acc_over, const long long *__restrict__ step_dev)
 {
     extern __shared__ int ts[];
-    __shared__ double red[(BS / kWave) * kAccK];
+    __shared__ double red[(BS / kWave) * 4];
     const int T = mp.n_traps;
     load_table(ts, trap_start, T + 1);
     const int n = *n_dev;
@@ -328,9 +461,12 @@ __global__ __launch_bounds__(BS) void k_move_fused(
     // traps covered by this block: usually one, so no per-thread search
     const int t_first = b0 < n ? last_le(ts, T, b0) : 0;
     const int t_last = b0 < n ? last_le(ts, T, min(n, b0 + BS) - 1) : 0;
-    double vals[kAccK];
-#pragma unroll
-    for (int q = 0; q < kAccK; ++q) vals[q] = 0.0;
+    // per atom: sum v_x^2, v_y^2, v_z^2 and U of survivors (kinetic energy and
+    // the alive count follow from them and the trap ranges), and the energy of
+    // evaporated atoms
+    double vals[4] = { 0.0, 0.0, 0.0, 0.0 };
+    double lost_e = 0.0;
+    bool lost = false;
     int t = -1;
     if (i < n) {
         t = t_first;
@@ -394,7 +530,6 @@ __global__ __launch_bounds__(BS) void k_move_fused(
                      VZ = (double)vz + (double)cvz;
         const double ek = 0.5 * mp.mass * (VX * VX + VY * VY + VZ * VZ);
         const double ep = potential_energy(tr, X, Y, Z);
-        bool lost = false;
         if (mp.spin_loss) {
             F3 Bn;
             (void)field_dir_f(tr, x, y, z, Bn);
@@ -412,14 +547,11 @@ __global__ __launch_bounds__(BS) void k_move_fused(
             evap_pos[3 * (size_t)id + 2] = (float)Z;
             evap_vel[3 * (size_t)id + 0] = (float)VX; evap_vel[3 * (size_t)id + 1] = (float)VY;
             evap_vel[3 * (size_t)id + 2] = (float)VZ;
-            vals[kAccEevap] = ek + ep;
-            vals[kAccNevap] = 1.0;
+            lost_e = ek + ep;
         } else {
-            kout = (uint32_t)(tr.key_base + cell_of(tr, X, Y, Z));
-            vals[kAccEk] = ek;
-            vals[kAccEp] = ep;
-            vals[kAccV2x] = VX * VX; vals[kAccV2y] = VY * VY; vals[kAccV2z] = VZ * VZ;
-            vals[kAccN] = 1.0;
+            kout = (uint32_t)(tr.key_base + cell_of_fast(tr, X, Y, Z));
+            vals[0] = VX * VX; vals[1] = VY * VY; vals[2] = VZ * VZ;
+            vals[3] = ep;
         }
         B.p[0][i] = x; B.p[1][i] = y; B.p[2][i] = z;
         B.cp[0][i] = cx; B.cp[1][i] = cy; B.cp[2][i] = cz;
@@ -436,22 +568,30 @@ __global__ __launch_bounds__(BS) void k_move_fused(
     // later traps go to acc_over by atomics.  Only the block holding a trap's
     // first slot can do that, so each accumulator receives at most one term
     // and the totals do not depend on the order of execution.
+    const bool any_lost = __syncthreads_or(lost);
     if (b0 < n) {
         const int nseg = t_last - t_first + 1;
         for (int sg = 0; sg < nseg; ++sg) {
             const int tt = t_first + sg;
-            double v[kAccK];
+            double v[4], e[4];
 #pragma unroll
-            for (int q = 0; q < kAccK; ++q) v[q] = (t == tt) ? vals[q] : 0.0;
-            block_sum<BS, kAccK>(v, red);
+            for (int q = 0; q < 4; ++q) v[q] = (t == tt) ? vals[q] : 0.0;
+            block_sum<BS, 4>(v, red);
+            e[0] = e[1] = e[2] = e[3] = 0.0;
+            if (any_lost) {
+                e[0] = (t == tt && lost) ? lost_e : 0.0;
+                e[1] = (t == tt && lost) ? 1.0 : 0.0;
+                block_sum<BS, 4>(e, red);
+            }
             if (threadIdx.x == 0) {
+                const double r[kAccK] = { 0.0, v[3], v[0], v[1], v[2], e[0], e[1], 0.0 };
                 if (sg == 0) {
 #pragma unroll
-                    for (int q = 0; q < kAccK; ++q) rec_sum[(size_t)q * rec_stride + blockIdx.x] = v[q];
+                    for (int q = 0; q < kAccK; ++q) rec_sum[(size_t)q * rec_stride + blockIdx.x] = r[q];
                     rec_trap[blockIdx.x] = t_first;
                 } else {
                     for (int q = 0; q < kAccK; ++q)
-                        if (v[q] != 0.0) atomicAdd(&acc_over[tt * kAccK + q], v[q]);
+                        if (r[q] != 0.0) atomicAdd(&acc_over[tt * kAccK + q], r[q]);
                 }
             }
         }
@@ -463,7 +603,7 @@ __global__ __launch_bounds__(BS) void k_move_fused(
 // adds the boundary term, appends the trap's observables to the log and
 // clears the per-step accumulators; the last trap advances the step.
 __global__ __launch_bounds__(256) void k_obs_finalize(
-    int T, int P, int move_block, const int *__restrict__ trap_start,
+    int T, int P, int move_block, double half_mass, const int *__restrict__ trap_start,
     const int *__restrict__ rec_trap, const double *__restrict__ rec_sum, int rec_stride,
     double *__restrict__ lvl2, unsigned int *__restrict__ cnt, double *__restrict__ acc_over,
     unsigned long long *__restrict__ coll_accum, unsigned int *__restrict__ done,
@@ -513,11 +653,11 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
     const unsigned long long c = coll_accum[t];
     coll_accum[t] = 0ull;
     dsmc_observables &o = obs_log[(step % obs_cap) * T + t];
-    o.ek = f[kAccEk];
+    o.ek = half_mass * (f[kAccV2x] + f[kAccV2y] + f[kAccV2z]);
     o.ep = f[kAccEp];
     o.v2[0] = f[kAccV2x]; o.v2[1] = f[kAccV2y]; o.v2[2] = f[kAccV2z];
     o.e_evap = f[kAccEevap];
-    o.n_alive = (long long)f[kAccN] + (long long)f[kAccNevap];
+    o.n_alive = (long long)(s1 - s0);
     o.n_evap = (long long)f[kAccNevap];
     o.collisions = (long long)c;
     o.step = step;
@@ -581,7 +721,8 @@ void launch_table_fused(const uint32_t *key, const int *key_base, const TablePar
 {
     const size_t lds = sizeof(int) * (size_t)(tp.n_traps + 1);
     if (prof && prof->enabled) prof->begin(st, "table_fused", (double)n_cap * 4.0);
-    hipLaunchKernelGGL(k_table_fused, dim3(fgrid((long long)n_cap + 1, kFBlock)), dim3(kFBlock),
+    hipLaunchKernelGGL(k_table_fused, dim3(fgrid((long long)n_cap + 1, kFBlock * kTableItems)),
+                       dim3(kFBlock),
                        lds, st, key, key_base, tp, n_in, n_out, trap_start);
     DSMC_HIP(hipGetLastError());
     if (prof && prof->enabled) prof->end(st);
@@ -602,10 +743,14 @@ void launch_collide_fused(const TrapDev *traps, const int *key_base, int n_traps
                           hipStream_t st, Profiler *prof)
 {
     const size_t lds = sizeof(int) * (size_t)(n_traps + 1);
+    // DSMC_COLLIDE_WAVE_TRIALS: trial count above which a cell is run by a
+    // whole wave (results are the same either way)
+    const char *env = std::getenv("DSMC_COLLIDE_WAVE_TRIALS");
+    const int wave_trials = env ? std::atoi(env) : kWaveTrialsDefault;
     if (prof && prof->enabled) prof->begin(st, "collide_fused", (double)n_cap * 4.0);
     hipLaunchKernelGGL(k_collide_fused, dim3(fgrid(n_cap, kFBlock * kCollItems)), dim3(kFBlock), lds,
                        st, traps, key_base, n_traps, cp, key, perm, n_dev, s, sigvrmax, coll_accum,
-                       step_dev);
+                       step_dev, wave_trials);
     DSMC_HIP(hipGetLastError());
     if (prof && prof->enabled) prof->end(st);
 }
@@ -660,7 +805,8 @@ void launch_move_fused(int block, const TrapDev *traps, const MoveParams &mp, co
     if (prof && prof->enabled) prof->end(st);
 }
 
-void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, const int *trap_start,
+void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, double mass,
+                         const int *trap_start,
                          const ObsRecords &rec, unsigned long long *coll_accum, unsigned int *done,
                          long long *step_dev, dsmc_observables *obs_log, int obs_cap,
                          hipStream_t st, Profiler *prof)
@@ -668,7 +814,7 @@ void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, const 
     const int blocks = div_up(max_trap_atoms > 0 ? max_trap_atoms : 1, move_block) + 1;
     const int P = std::min(kMaxFinalizeP, div_up(blocks, 2048));
     DSMC_LAUNCH(prof, "obs_finalize", (double)n_traps * 64.0, k_obs_finalize, dim3(P, n_traps), 256,
-                st, n_traps, P, move_block, trap_start, (const int *)rec.trap,
+                st, n_traps, P, move_block, 0.5 * mass, trap_start, (const int *)rec.trap,
                 (const double *)rec.sum, rec.stride, rec.lvl2, rec.cnt, rec.acc_over, coll_accum,
                 done, step_dev, obs_log, obs_cap);
 }

@@ -78,7 +78,8 @@ void launch_move_fused(int block, const TrapDev *traps, const MoveParams &mp, co
                        const int *n_dev, uint32_t *key_out, float *evap_pos, float *evap_vel,
                        const ObsRecords &rec, const long long *step_dev, int n_cap, hipStream_t st,
                        Profiler *prof);
-void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, const int *trap_start,
+void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, double mass,
+                         const int *trap_start,
                          const ObsRecords &rec, unsigned long long *coll_accum, unsigned int *done,
                          long long *step_dev, dsmc_observables *obs_log, int obs_cap,
                          hipStream_t st, Profiler *prof);

@@ -96,6 +96,31 @@ def test_fused_energy_conservation(n, nc, steps):
     assert coll > 0
 
 
+@pytest.mark.parametrize("pair_mode", ["uniform", "reference"])
+def test_wave_collide_equals_serial(pair_mode, monkeypatch):
+    """Cells above the wave-trial threshold are run by a whole wave
+    speculatively; the result must be bitwise that of one lane per cell.
+    Dense cells (n_real = 3e9 on 1e5 atoms, 12^3 cells) give hundreds of
+    trials per cell."""
+    trap = configs.quadrupole_trap(100000, 12, n_real=3e9)
+    run = configs.Run(traps=[trap], precision="fp32", dt=1e-6, seed=5, pair_mode=pair_mode,
+                      name="wave-collide")
+    out = {}
+    for thr in ("1000000000", "0", "16"):
+        monkeypatch.setenv("DSMC_COLLIDE_WAVE_TRIALS", thr)
+        print(f"wave collide threshold {thr}, pair mode {pair_mode}", flush=True)
+        with Simulation(run) as sim:
+            sim.init_distribution()
+            sim.step(20)
+            out[thr] = (sim.copy_out(_abi.FIELD_POS), sim.copy_out(_abi.FIELD_VEL),
+                        sim.obs_log()["collisions"].copy())
+    ser = out["1000000000"]
+    assert int(ser[2].sum()) > 20 * 1000
+    for thr in ("0", "16"):
+        for a, b in zip(ser, out[thr]):
+            np.testing.assert_array_equal(a, b)
+
+
 def test_graph_replay_equals_eager():
     run = quad(40000, 30)
     with Simulation(run) as a, Simulation(run.with_(use_graph=False)) as b:
