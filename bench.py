@@ -9,16 +9,24 @@ time of exactly --steps steps on the context's stream), state resident in HBM.
 
 Also reported: the FP64 plain (drop-in) form and the FP32 form on the single
 quadrupole trap N = 1e6 / 100^3 cells (the precision-form workload), their
-ratio, the FP64 energy drift of the FP32 run, a per-kernel roofline from
-hipEvents around every launch of a separate profiled run, and the CPU oracle
-(OpenMP FP64 restatement of the reference kernels) on N = 1e4 / 35^3.
+ratio, the FP64 energy drift of the FP32 runs, a per-kernel roofline from
+hipEvents around every launch of a profiled pass of the same workload, the
+HBM traffic of the dominant kernel from two rocprofv3 counter runs (FETCH_SIZE,
+then WRITE_SIZE) scaled by what the same counters read for a streaming copy of
+known size, and the CPU oracle (OpenMP FP64 restatement of the reference
+kernels) on N = 1e4 / 35^3.
 """
 from __future__ import annotations
 
 import argparse
+import glob
 import json
 import os
+import shutil
+import sqlite3
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -68,9 +76,11 @@ def run_gpu_leg(run, steps, warmup, autotune=True):
 
 
 def energy_drift(sim, steps):
-    lg = sim.obs_log()[-steps:, 0]
-    E = lg["ek"] + lg["ep"] + np.cumsum(lg["e_evap"])
-    return float(np.max(np.abs(E - E[0])) / abs(E[0]))
+    """max over traps of max_t |E(t) - E(0)| / E(0), E = alive energy plus the
+    energy carried away by the atoms removed so far."""
+    lg = sim.obs_log()[-steps:]
+    E = lg["ek"] + lg["ep"] + np.cumsum(lg["e_evap"], axis=0)
+    return float(np.max(np.abs(E - E[0]) / np.abs(E[0])))
 
 
 def profile_leg(sim, steps):
@@ -118,6 +128,81 @@ def copy_calibration(mbytes=1024, repeats=20):
     return dict(bytes=nbytes, ms=ms.value, GBs=2.0 * nbytes / (ms.value * 1e-3) / 1e9)
 
 
+def pmc_child(steps):
+    """Body of the counter runs: a streaming copy of known size, then `steps`
+    eager steps of the batched workload."""
+    from cudadsmc_amd import Simulation, configs
+
+    cal = copy_calibration(mbytes=1024, repeats=3)
+    run = configs.batched_traps().with_(use_graph=False, obs_capacity=steps + 64)
+    with Simulation(run) as sim:
+        sim.init_distribution()
+        sim.step(steps)
+        sim.sync()
+    print(json.dumps({"copy_bytes": cal["bytes"]}), flush=True)
+
+
+def _pmc_values(db_glob):
+    """Median counter value per dispatch, by kernel name, from a rocpd database
+    (the median drops the first step after initialisation, whose gather runs
+    in random order)."""
+    out = {}
+    for db in glob.glob(db_glob, recursive=True):
+        con = sqlite3.connect(db)
+        rows = con.execute("select kernel_name, dispatch_id, sum(value) from counters_collection "
+                           "group by kernel_name, dispatch_id").fetchall()
+        con.close()
+        for name, _, v in rows:
+            out.setdefault(name, []).append(float(v))
+    return {k: float(np.median(v)) for k, v in out.items()}
+
+
+def pmc_traffic(steps=10, timeout_s=150):
+    """HBM bytes per launch of move_fused: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) of two rocprofv3 runs, each scaled by the ratio of the known
+    bytes of the streaming copy to what the same counter reads for it.
+    Runs before this process touches the GPU; None when rocprofv3 is absent
+    or a run fails."""
+    prof = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3"
+                                          if os.path.exists("/opt/rocm/bin/rocprofv3") else None)
+    if prof is None:
+        return None
+    tmp = tempfile.mkdtemp(prefix="dsmc_pmc_")
+    vals = {}
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = ["timeout", "-s", "KILL", str(timeout_s), prof, "--pmc", counter,
+                   "-d", d, "-o", "pmc", "--",
+                   sys.executable, str(ROOT / "bench.py"), "--pmc-child", str(steps)]
+            log(f"[bench] counter run {counter}")
+            res = subprocess.run(cmd, capture_output=True, text=True)
+            v = _pmc_values(os.path.join(d, "**", "*.db"))
+            if not v:
+                log(f"[bench] counter run {counter} gave no data (rc {res.returncode})")
+                return None
+            vals[counter] = v
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+    def pick(v, key):
+        names = [k for k in v if key in k]
+        return v[names[0]] if names else None
+
+    copy_bytes = 1024 << 20
+    f_copy, w_copy = pick(vals["FETCH_SIZE"], "stream_copy"), pick(vals["WRITE_SIZE"], "stream_copy")
+    f_move, w_move = pick(vals["FETCH_SIZE"], "move_fused"), pick(vals["WRITE_SIZE"], "move_fused")
+    if None in (f_copy, w_copy, f_move, w_move) or f_copy <= 0 or w_copy <= 0:
+        return None
+    f_scale = copy_bytes / (f_copy * 1024.0)
+    w_scale = copy_bytes / (w_copy * 1024.0)
+    fetch = f_move * 1024.0 * f_scale
+    write = w_move * 1024.0 * w_scale
+    return dict(kernel="move_fused", fetch_bytes=fetch, write_bytes=write, bytes=fetch + write,
+                fetch_scale=round(f_scale, 4), write_scale=round(w_scale, 4),
+                fetch_kib_raw=f_move, write_kib_raw=w_move, steps=steps)
+
+
 def cpu_baseline(target_s=12.0):
     from cudadsmc_amd import configs
     from oracle import oracle as orc
@@ -148,12 +233,24 @@ def main():
     ap.add_argument("--legs", default="batched,fp64,fp32")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the streaming-copy bandwidth calibration")
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the rocprofv3 counter runs (roofline.traffic = null)")
+    ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs one GPU: the DSMC step is measured on a single MI355X; "
-              "multi-GPU sharding is a design note only (see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the DSMC step runs on a single MI355X and "
+              "runs over several GPUs are not supported", file=sys.stderr)
         sys.exit(2)
+
+    if args.pmc_child:
+        pmc_child(args.pmc_child)
+        return
+    legs = set(args.legs.split(","))
+    # the counter runs are child processes started before this one uses the GPU
+    traffic = None
+    if not args.no_pmc and not args.copy_only and "batched" in legs:
+        traffic = pmc_traffic()
 
     from cudadsmc_amd import configs, load_library
 
@@ -161,7 +258,6 @@ def main():
     if args.copy_only:
         print(json.dumps({"copy_calibration": copy_calibration()}), flush=True)
         return
-    legs = set(args.legs.split(","))
     result = {}
 
     batched = configs.batched_traps(steps=args.steps)
@@ -171,7 +267,9 @@ def main():
     if "batched" in legs:
         log(f"[bench] batched: {len(batched.traps)} traps, {batched.n_atoms} atoms")
         sim, b = run_gpu_leg(batched, args.steps, args.warmup)
-        log(f"[bench] batched FP32: {b['value']:.4e} atom-steps/s ({b['ms']:.1f} ms, block {b['block']})")
+        b["energy_drift"] = energy_drift(sim, args.steps)
+        log(f"[bench] batched FP32: {b['value']:.4e} atom-steps/s ({b['ms']:.1f} ms, block {b['block']}), "
+            f"max |dE/E| {b['energy_drift']:.2e}")
         result["batched"] = b
         kernels, n_mean = profile_leg(sim, args.profile_steps)
         info = sim.info()
@@ -180,10 +278,15 @@ def main():
                   key=lambda k: kernels[k]["total_ms"])
         d = kernels[dom]
         achieved = d["alg_bytes"] / (d["avg_us"] * 1e-6) / 1e9
+        tr = traffic if (traffic and traffic["kernel"] == dom) else None
         roof = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=round(tr["bytes"]) if tr else None,
                     alg_bytes_per_launch=round(d["alg_bytes"]), avg_launch_us=d["avg_us"],
                     radix_passes=info.radix_passes)
+        if tr:
+            roof["traffic_detail"] = {k: (round(v, 1) if isinstance(v, float) else v)
+                                      for k, v in tr.items()}
         log(f"[bench] dominant kernel {dom}: {achieved:.0f} GB/s")
     if "fp64" in legs:
         run64 = configs.quadrupole_1e6("fp64").with_(obs_capacity=args.single_steps + args.warmup + 64)
@@ -229,6 +332,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if "batched" in result:
+        line["energy_drift"] = result["batched"]["energy_drift"]
     if "fp64_plain" in result:
         line["fp64_plain"] = {k: round(v, 3) if isinstance(v, float) else v
                               for k, v in result["fp64_plain"].items()}

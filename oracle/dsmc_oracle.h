@@ -23,7 +23,7 @@
  *   - cell length, grid extent, dt, loops-per-collision, magnetic moment and
  *     trap kind are run-time parameters instead of compile-time constants.
  *
- * Parity pins for each function are listed in tests/test_oracle_*.py.
+ * Parity pins for each function are in tests/test_oracle.py.
  */
 #ifndef DSMC_ORACLE_H
 #define DSMC_ORACLE_H

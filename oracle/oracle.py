@@ -31,6 +31,10 @@ class Params(C.Structure):
     ]
 
 
+class D3(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+
 class RunStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("atom_steps", C.c_longlong),
                 ("collisions", C.c_longlong), ("threads", C.c_int)]
@@ -67,6 +71,8 @@ def load():
             "orc_sigvrmax_init": (C.c_double, [P, C.c_double]),
             "orc_run": (None, [P, P, P, P, P, P, P, C.c_int, C.c_int, C.c_uint32, P]),
             "orc_num_threads": (C.c_int, []),
+            "orc_field": (D3, [P, D3]),
+            "orc_potential": (C.c_double, [P, D3]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(_lib, name)
@@ -115,6 +121,17 @@ def u01(hi, lo):
 
 def uint_below(r, n):
     return load().orc_uint_below(r, n)
+
+
+def field(p: Params, r):
+    """B(r) of the trap (T)."""
+    b = load().orc_field(C.byref(p), D3(*map(float, r)))
+    return np.array([b.x, b.y, b.z])
+
+
+def potential(p: Params, r):
+    """U(r) of one atom (J)."""
+    return load().orc_potential(C.byref(p), D3(*map(float, r)))
 
 
 def ncells(p: Params) -> int:

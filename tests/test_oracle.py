@@ -415,3 +415,166 @@ def test_quadrupole_initial_distribution_moments(orc):
     # isotropy of the direction in (x, y, 2z)
     u = np.stack([x, y, 2 * z], 1) / rho[:, None]
     assert np.all(np.abs(u.mean(0)) < 0.02)
+
+
+# ---------------------------------------------------------------------------
+# Field: Maxwell's equations in vacuum (div B = 0, curl B = 0)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", ["ip", "quadrupole"])
+def test_field_is_divergence_and_curl_free(orc, kind):
+    """A sign or index slip in either field (moveAtoms.cu:135-142, or the
+    quadrupole B' (x, y, -2z)) breaks div B = 0 or curl B = 0; |B| is the
+    magnitude the potential uses (U = mu (|B| - B0))."""
+    run = configs.reference() if kind == "ip" else configs.quadrupole_1e4()
+    p = orc.params_from(run)
+    t = run.traps[0]
+    rng = np.random.default_rng(7)
+    h = 1e-9
+    for _ in range(25):
+        r = rng.normal(0, 5e-5, 3)
+        J = np.zeros((3, 3))  # J[i, j] = dB_i / dx_j
+        for j in range(3):
+            e = np.zeros(3); e[j] = h
+            J[:, j] = (orc.field(p, r + e) - orc.field(p, r - e)) / (2 * h)
+        scale = np.abs(J).max()
+        assert abs(np.trace(J)) < 1e-6 * scale
+        curl = np.array([J[2, 1] - J[1, 2], J[0, 2] - J[2, 0], J[1, 0] - J[0, 1]])
+        assert np.all(np.abs(curl) < 1e-6 * scale)
+        B = orc.field(p, r)
+        b0 = t.B0 if kind == "ip" else 0.0
+        assert orc.potential(p, r) == pytest.approx(t.mu * (np.linalg.norm(B) - b0), rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Evaporation tag, temperature, energy
+# ---------------------------------------------------------------------------
+def test_evaporation_tag_spin_projection_and_energy_cut(orc):
+    """evaporation.cu:90-94: an atom is lost when its spin points against B.
+    Spinors aligned with B (setUp.cu:236-252) survive; the orthogonal spinor
+    (-conj d, conj u) has the opposite Bloch vector and is lost.  Energy cut:
+    lost iff 1/2 m v^2 + U > E_cut, with U from the independent potential;
+    lost atoms have their position and velocity recorded."""
+    run = configs.reference(20, 400)
+    p = orc.params_from(run)
+    st = orc.init_distribution(p, 400, 20e-6, spin=True)
+    ids = np.arange(400, dtype=np.int32)
+    tag, _, _ = orc.evaporation_tag(p, st["pos"], st["vel"], ids, st["psi_up"], st["psi_dn"])
+    assert not tag.any()
+    u, d = st["psi_up"], st["psi_dn"]
+    flip = np.arange(400) % 3 == 0
+    u2, d2 = u.copy(), d.copy()
+    u2[flip] = np.stack([-d[flip, 0], d[flip, 1]], 1)   # -conj(d)
+    d2[flip] = np.stack([u[flip, 0], -u[flip, 1]], 1)   # conj(u)
+    tag, epos, evel = orc.evaporation_tag(p, st["pos"], st["vel"], ids, u2, d2)
+    np.testing.assert_array_equal(tag.astype(bool), flip)
+    np.testing.assert_array_equal(epos[flip], st["pos"][flip])
+    np.testing.assert_array_equal(evel[flip], st["vel"][flip])
+    assert np.all(epos[~flip] == 0.0)
+    # energy cut alone
+    E = np.array([0.5 * run.mass * (v @ v) + _potential(run, r) for r, v in zip(st["pos"], st["vel"])])
+    cut = float(np.median(E))
+    margin = np.abs(E - cut) > 1e-9 * cut
+    tag, _, _ = orc.evaporation_tag(p, st["pos"], st["vel"], ids, e_cut=cut)
+    np.testing.assert_array_equal(tag.astype(bool)[margin], (E > cut)[margin])
+
+
+def test_temperature_energy_and_virial(orc):
+    """calculateTemp (evaporation.cu:108-128) on equal speeds gives
+    m v^2 / (3 kB) exactly; on a thermal cloud T -> T0.  The quadrupole
+    potential is homogeneous of degree 1, so in equilibrium <U> = 3 kT
+    (= 2 <Ek>) -- a relation between the sampler and the potential."""
+    run = configs.quadrupole_1e4()
+    t = run.traps[0]
+    p = orc.params_from(run)
+    n = 1000
+    rng = np.random.default_rng(2)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    v = np.ascontiguousarray(0.03 * dirs)
+    ids = np.arange(n, dtype=np.int32)
+    T = orc.calculate_temp(p, v, ids)
+    assert T == pytest.approx(run.mass * 0.03 ** 2 / (3 * configs.KB), rel=1e-13)
+    ek, ep, v2 = orc.energy(p, np.zeros((n, 3)), v, ids)
+    assert ek == pytest.approx(n * 0.5 * run.mass * 0.03 ** 2, rel=1e-13)
+    assert ep == 0.0 and v2 == pytest.approx(n * 0.03 ** 2, rel=1e-13)
+    st = orc.init_distribution(p, 100000, t.T)
+    ids = np.arange(100000, dtype=np.int32)
+    kT = configs.KB * t.T
+    assert orc.calculate_temp(p, st["vel"], ids) == pytest.approx(t.T, rel=0.01)
+    ek, ep, _ = orc.energy(p, st["pos"], st["vel"], ids)
+    assert ep / 100000 == pytest.approx(3 * kT, rel=0.01)
+    assert ep / ek == pytest.approx(2.0, rel=0.015)
+
+
+def test_sigvrmax_init_is_mean_relative_speed(orc):
+    """initSigvrmax (setUp.cu:287-292): sqrt(16 kT / (pi m)) sigma is sigma
+    times the mean relative speed of two Maxwellian atoms; check against
+    sampled pairs of the initial velocities."""
+    run = configs.quadrupole_1e4()
+    t = run.traps[0]
+    p = orc.params_from(run)
+    st = orc.init_distribution(p, 200000, t.T)
+    v = st["vel"]
+    vr = np.linalg.norm(v[:100000] - v[100000:], axis=1).mean()
+    assert orc.sigvrmax_init(p, t.T) == pytest.approx(vr * run.sigma, rel=0.01)
+
+
+def test_ip_initial_distribution_matches_rejection_density(orc):
+    """The rejection sampler of setUp.cu:178-205 proposes from a Gaussian of
+    width W/3 per axis and accepts with probability exp(-U / kT) (capped at
+    1 where U < 0, in the field zeros of the IP expansion), so positions
+    follow q(r) min(1, exp(-U / kT)).  Independent check: importance-weighted
+    numpy proposals with the independently written IP potential; the
+    quantiles of |x|, |y|, |z| must agree."""
+    run = configs.reference(20, 20000)
+    t = run.traps[0]
+    p = orc.params_from(run)
+    st = orc.init_distribution(p, 20000, t.T)
+    kT = configs.KB * t.T
+    rng = np.random.default_rng(11)
+    m = 4_000_000
+    X, Y, Z = (rng.normal(0.0, hw / 3.0, m) for hw in t.half_width)
+    Bx = t.dBdx * X - 0.5 * t.d2Bdx2 * X * Z
+    By = -t.dBdx * Y - 0.5 * t.d2Bdx2 * Y * Z
+    Bz = t.B0 + 0.5 * t.d2Bdx2 * (Z * Z - 0.5 * (X * X + Y * Y))
+    U = t.mu * (np.sqrt(Bx * Bx + By * By + Bz * Bz) - t.B0)
+    w = np.minimum(1.0, np.exp(-U / kT))
+    for q, Q in zip(st["pos"].T, (X, Y, Z)):
+        order = np.argsort(np.abs(Q))
+        cdf = np.cumsum(w[order]) / w.sum()
+        for f in (0.25, 0.5, 0.75, 0.9, 0.99):
+            expect = float(np.abs(Q)[order][np.searchsorted(cdf, f)])
+            assert float(np.quantile(np.abs(q), f)) == pytest.approx(expect, rel=0.035)
+
+
+def test_box_initial_distribution(orc):
+    """Periodic box (homogeneous-gas configuration): positions uniform over
+    the box, per-axis Maxwell velocities at Tx = 2 Ty = 2 Tz."""
+    from scipy import stats
+    run = configs.box_thermalise("fp64", n_atoms=50000, nc=16)
+    t = run.traps[0]
+    p = orc.params_from(run)
+    st = orc.init_distribution(p, 50000, t.T, t.T_axis)
+    for d in range(3):
+        hw = t.half_width[d]
+        assert stats.kstest((st["pos"][:, d] + hw) / (2 * hw), "uniform").pvalue > 1e-3
+        s = math.sqrt(configs.KB * t.T_axis[d] / run.mass)
+        assert stats.kstest(st["vel"][:, d] / s, "norm").pvalue > 1e-3
+
+
+def test_run_loop_equals_step_composition(orc):
+    """orc_run (the CPU baseline loop) is exactly index -> sort -> table ->
+    collide -> lpc x move, the composition the Reference class performs."""
+    run = configs.quadrupole_1e4().with_(lpc=2)
+    t = run.traps[0]
+    ref = orc.Reference(run.with_(traps=[configs.quadrupole_trap(3000, 20)]))
+    p = ref.p
+    st = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in ref.s.items()}
+    sv = ref.sigvrmax.copy()
+    res = orc.run(p, st, 3, 0, sv)
+    ref.step(6)
+    np.testing.assert_array_equal(st["pos"], ref.s["pos"])
+    np.testing.assert_array_equal(st["vel"], ref.s["vel"])
+    np.testing.assert_array_equal(sv, ref.sigvrmax)
+    assert res["collisions"] == ref.collisions and res["atom_steps"] == 6 * 3000
+    del t
