@@ -19,6 +19,7 @@ STATUS_NAMES = {0: "DSMC_OK", 1: "DSMC_EINVAL", 2: "DSMC_EHIP", 3: "DSMC_ENOMEM"
 TRAP_IP, TRAP_QUADRUPOLE, TRAP_BOX = 0, 1, 2
 FP64_PLAIN, FP32_KAHAN = 0, 1
 PAIR_UNIFORM, PAIR_REFERENCE = 0, 1
+SORT_INCREMENTAL, SORT_FULL = 0, 1
 EVAP_SPIN, EVAP_ENERGY = 1, 2
 
 (FIELD_POS, FIELD_VEL, FIELD_ACC, FIELD_ATOM_ID, FIELD_CELL_ID, FIELD_PERM, FIELD_SPIN,
@@ -46,7 +47,7 @@ class Config(C.Structure):
         ("lpc", C.c_int), ("use_graph", C.c_int),
         ("dt", C.c_double), ("mass", C.c_double), ("sigma", C.c_double), ("gs", C.c_double),
         ("seed", C.c_ulonglong), ("block_size", C.c_int), ("obs_capacity", C.c_int),
-        ("reserved", C.c_int * 6),
+        ("sort_mode", C.c_int), ("reserved", C.c_int * 5),
     ]
 
 

@@ -104,6 +104,7 @@ class Run:
     use_graph: bool = True
     block_size: int = 0
     obs_capacity: int = 0
+    sort_mode: str = "incremental"     # or "full": radix sort of every key each step
     steps: int = 1000
     name: str = "custom"
 
@@ -132,6 +133,7 @@ class Run:
         c.seed = int(self.seed)
         c.block_size = int(self.block_size)
         c.obs_capacity = int(self.obs_capacity)
+        c.sort_mode = _abi.SORT_INCREMENTAL if self.sort_mode == "incremental" else _abi.SORT_FULL
         return c
 
 

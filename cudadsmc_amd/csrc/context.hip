@@ -120,6 +120,8 @@ struct dsmc_ctx {
     int obs_cap = 0;
     int block = 256;
     bool have_sort = false;         // FP32: perm valid for the next move
+    bool srt_valid = false;         // FP32: key_b[i] is the key of slot i of the current
+                                    // state (its slots are in sorted order of key_b)
     std::vector<void *> allocs;
 
     // common device buffers
@@ -128,6 +130,7 @@ struct dsmc_ctx {
     uint32_t *key_a = nullptr, *key_b = nullptr, *val_a = nullptr, *val_b = nullptr;
     const uint32_t *sorted_key = nullptr, *perm = nullptr;
     SortScratch sort;
+    ResortScratch res;              // FP32: incremental re-sort
     double *sigvrmax = nullptr;
     dsmc_observables *obs_log = nullptr;  // (obs_cap + 1) * T records
     long long *step_dev = nullptr;
@@ -253,6 +256,8 @@ static void validate(const dsmc_config *c, const dsmc_trap *traps, int n_traps)
         fail(DSMC_EINVAL, "spin-loss evaporation needs spin evolution");
     if (c->pair_mode != DSMC_PAIR_UNIFORM && c->pair_mode != DSMC_PAIR_REFERENCE)
         fail(DSMC_EINVAL, "bad pair_mode");
+    if (c->sort_mode != DSMC_SORT_INCREMENTAL && c->sort_mode != DSMC_SORT_FULL)
+        fail(DSMC_EINVAL, "bad sort_mode");
     const int bs = c->block_size;
     if (bs != 0 && bs != 64 && bs != 128 && bs != 256 && bs != 512 && bs != 1024)
         fail(DSMC_EINVAL, "block_size must be 0, 64, 128, 256, 512 or 1024");
@@ -365,6 +370,7 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
             }
             x->fs[b].id = x->alloc<int>(N);
         }
+        resort_scratch_bind(x->res, N, x->alloc<char>(resort_scratch_bytes(N), false));
         x->cell_start = x->alloc<uint32_t>(x->n_keys, false);
         x->cell_end = x->alloc<uint32_t>(x->n_keys, false);
         x->trap_start = x->alloc<int>(n_traps + 1, false);
@@ -536,10 +542,23 @@ static void fused_index(dsmc_ctx *x)
                        x->n_cap, x->stream, &x->prof);
 }
 
+// Stable sort of the keys in key_a into (key_b, val_b).  When the current
+// state is in sorted order of key_b (after a sorted move), only the atoms
+// whose key changed are sorted and merged into the rest; the result is the
+// same permutation.
 static void fused_sort(dsmc_ctx *x)
 {
-    run_sort(x, x->n_dev + x->cur);
+    const int *n = x->n_dev + x->cur;
+    if (x->srt_valid && x->cfg.sort_mode == DSMC_SORT_INCREMENTAL)
+        resort_incremental(x->key_a, x->key_b, n, x->key_bits, x->res, x->sort, x->key_b,
+                           x->val_b, x->stream, &x->prof);
+    else
+        radix_sort(x->key_a, nullptr, x->res.alt_k, x->res.alt_v, x->key_b, x->val_b, n,
+                   x->key_bits, x->sort, x->stream, &x->prof);
+    x->sorted_key = x->key_b;
+    x->perm = x->val_b;
     x->have_sort = true;
+    x->srt_valid = false;   // the state is not yet in the new order
 }
 
 static void fused_table(dsmc_ctx *x)
@@ -572,6 +591,9 @@ static void fused_move(dsmc_ctx *x)
     if (!x->have_sort)  // keep the count with the state it describes
         DSMC_HIP(hipMemcpyAsync(x->n_dev + (x->cur ^ 1), x->n_dev + x->cur, sizeof(int),
                                 hipMemcpyDeviceToDevice, x->stream));
+    // a sorted move leaves the state in sorted order of key_b; an unsorted one
+    // keeps the slot order (and so the validity of key_b)
+    if (x->have_sort) x->srt_valid = true;
     x->have_sort = false;
     x->cur ^= 1;
 }
@@ -623,7 +645,10 @@ static void do_steps(dsmc_ctx *x, int n)
     if (x->have_sort) fail(DSMC_ESTATE, "a sort is pending: call dsmc_move_atoms first");
     int k = 0;
     if (graph_ok(x)) {
-        if (x->cur == 1 && n > 0) { fused_step(x); ++k; }
+        // the graph is captured from (and replayed at) state buffer 0 with a
+        // valid previous sort
+        const bool need_srt = x->cfg.sort_mode == DSMC_SORT_INCREMENTAL;
+        while (k < n && (x->cur == 1 || (need_srt && !x->srt_valid))) { fused_step(x); ++k; }
         const int pairs = (n - k) / 2;
         if (pairs > 0) { fused_graph_steps(x, pairs); k += 2 * pairs; }
     }
@@ -719,6 +744,7 @@ static void reset_cells(dsmc_ctx *x)
     x->coll_steps = 0;
     x->cur = 0;
     x->have_sort = false;
+    x->srt_valid = false;
 }
 
 static InitParams init_params(const dsmc_ctx *x)
@@ -1282,7 +1308,7 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
             DSMC_HIP(hipEventRecord(b, ctx->stream));
             for (int k = 0; k < 5; ++k)
                 launch_move_fused(bs, ctx->d_traps, mp, ctx->fs[ctx->cur], ctx->fs[ctx->cur ^ 1],
-                                  nullptr, ctx->trap_start, ctx->n_dev + ctx->cur, ctx->key_b,
+                                  nullptr, ctx->trap_start, ctx->n_dev + ctx->cur, ctx->res.alt_k,
                                   ctx->evap_pos_f, ctx->evap_vel_f, ctx->rec, ctx->step_dev,
                                   ctx->n_cap, ctx->stream, nullptr);
             DSMC_HIP(hipEventRecord(e, ctx->stream));

@@ -89,6 +89,22 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
                 const int *n_dev, int key_bits, const SortScratch &s,
                 hipStream_t stream, Profiler *prof);
 
+// Incremental re-sort (see primitives.hip): the stable sort of key_new[0..n)
+// by (key, slot), for slots that are in stable sorted order of key_old.
+struct ResortScratch {
+    int n_cap = 0, tiles = 0;
+    uint32_t *stay_k = nullptr, *stay_v = nullptr, *mov_k = nullptr, *mov_v = nullptr;
+    uint32_t *movs_k = nullptr, *movs_v = nullptr, *alt_k = nullptr, *alt_v = nullptr;
+    uint32_t *tile_cnt = nullptr;
+    int *split = nullptr;
+    int *counts = nullptr;       // [0] stay, [1] movers
+};
+size_t resort_scratch_bytes(int n_cap);
+void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem);
+void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const int *n_dev,
+                        int key_bits, const ResortScratch &r, const SortScratch &s,
+                        uint32_t *out_k, uint32_t *out_v, hipStream_t stream, Profiler *prof);
+
 // Exclusive scan of n ints (n known on the host); partial holds div_up(n, tile).
 void exclusive_scan(const int *in, int *out, int n, int *partial, hipStream_t stream,
                     Profiler *prof);

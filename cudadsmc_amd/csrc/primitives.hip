@@ -45,28 +45,33 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_upsweep(
             if (valid && (peers & lt) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
         }
     }
+    // tiles past the end are not scanned (block 0 always writes: n may be 0)
+    if (tile0 >= n && blockIdx.x > 0) return;
     __syncthreads();
     for (int d = tid; d < kRadix; d += kSortBlock) hist[(size_t)d * nb + blockIdx.x] = h[d];
 }
 
 // One block per digit: exclusive scan of that digit's per-block counts.
 __global__ __launch_bounds__(256) void k_radix_scan(uint32_t *__restrict__ hist,
-                                                    uint32_t *__restrict__ totals, int nb)
+                                                    uint32_t *__restrict__ totals,
+                                                    const int *__restrict__ n_dev, int nb)
 {
     __shared__ uint32_t tmp[256 / kWave + 1];
+    const int n = *n_dev;
+    const int nb_used = min(nb, ((n > 0 ? n : 1) + kSortTile - 1) / kSortTile);  // tiles with data
     uint32_t *row = hist + (size_t)blockIdx.x * nb;
-    const int per = (nb + 255) / 256;
+    const int per = (nb_used + 255) / 256;
     const int b0 = threadIdx.x * per;
     uint32_t s = 0;
     for (int i = 0; i < per; ++i) {
         const int j = b0 + i;
-        if (j < nb) s += row[j];
+        if (j < nb_used) s += row[j];
     }
     uint32_t total;
     uint32_t ex = block_exclusive_scan<256>(s, tmp, &total);
     for (int i = 0; i < per; ++i) {
         const int j = b0 + i;
-        if (j < nb) { const uint32_t c = row[j]; row[j] = ex; ex += c; }
+        if (j < nb_used) { const uint32_t c = row[j]; row[j] = ex; ex += c; }
     }
     if (threadIdx.x == 0) totals[blockIdx.x] = total;
 }
@@ -183,12 +188,233 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
         DSMC_LAUNCH(prof, "radix_upsweep", (double)s.n_cap * 4.0, k_radix_upsweep, s.nb,
                     kSortBlock, stream, sk, n_dev, shift, s.hist, s.nb);
         DSMC_LAUNCH(prof, "radix_scan", (double)kRadix * s.nb * 8.0, k_radix_scan, kRadix, 256,
-                    stream, s.hist, s.totals, s.nb);
+                    stream, s.hist, s.totals, n_dev, s.nb);
         DSMC_LAUNCH(prof, "radix_downsweep", bytes_pass, k_radix_downsweep, s.nb, kSortBlock,
                     stream, sk, sv, dk, dv, n_dev, shift, s.hist, s.totals, s.nb);
         sk = dk;
         sv = dv;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Incremental re-sort.  Input: keys key_new[0..n) of slots that were in
+// stable sorted order of key_old[0..n) (the previous sort's keys).  Output:
+// the stable sort of key_new by (key, slot) -- identical to a full radix sort
+// -- built as
+//   1. split: slots whose key did not change ("stay", already in order) and
+//      the others ("movers"), both in slot order;
+//   2. radix sort of the movers alone (stable, so ties stay in slot order);
+//   3. merge path of the two sorted runs by (key, slot).
+// Between time steps only atoms that crossed a cell face (a few percent)
+// are movers, so the full-width sort runs on a few percent of the data.
+// ---------------------------------------------------------------------------
+constexpr int kSplitTile = kSortTile;    // slots per split block (256 x 8)
+constexpr int kMergeTile = kSortTile;    // outputs per merge block
+constexpr int kMergeItems = kMergeTile / kSortBlock;
+
+__global__ __launch_bounds__(kSortBlock) void k_split_count(const uint32_t *__restrict__ key_new,
+                                                            const uint32_t *__restrict__ key_old,
+                                                            const int *__restrict__ n_dev,
+                                                            uint32_t *__restrict__ tile_cnt)
+{
+    __shared__ uint32_t tmp[kSortBlock / kWave + 1];
+    const int n = *n_dev;
+    const int tile0 = blockIdx.x * kSplitTile;
+    uint32_t c = 0;
+    if (tile0 < n) {
+#pragma unroll
+        for (int k = 0; k < kSortItems; ++k) {
+            const int i = tile0 + k * kSortBlock + threadIdx.x;
+            if (i < n) c += key_new[i] != key_old[i];
+        }
+    }
+    uint32_t total;
+    (void)block_exclusive_scan<kSortBlock>(c, tmp, &total);
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_split_scatter(
+    const uint32_t *__restrict__ key_new, const uint32_t *__restrict__ key_old,
+    const int *__restrict__ n_dev, const uint32_t *__restrict__ tile_cnt,
+    uint32_t *__restrict__ stay_k, uint32_t *__restrict__ stay_v, uint32_t *__restrict__ mov_k,
+    uint32_t *__restrict__ mov_v, int *__restrict__ counts)
+{
+    constexpr int W = kSortBlock / kWave;
+    __shared__ uint32_t tmp[W + 1];
+    __shared__ uint32_t wcnt[W];
+    const int n = *n_dev;
+    const int tile0 = blockIdx.x * kSplitTile;
+    const int last_tile = n > 0 ? (n - 1) / kSplitTile : 0;
+    if ((int)blockIdx.x > last_tile) return;  // whole block leaves together
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    // movers in earlier tiles (each block sums the counts itself: no waiting)
+    uint32_t before = 0;
+    for (int t = tid; t < (int)blockIdx.x; t += kSortBlock) before += tile_cnt[t];
+    uint32_t tot;
+    (void)block_exclusive_scan<kSortBlock>(before, tmp, &tot);
+    before = tot;
+    // each wave owns a contiguous run of kWave * kSortItems slots
+    const int base = tile0 + w * (kWave * kSortItems);
+    const uint64_t lt = lanemask_lt();
+    uint32_t kn[kSortItems];
+    bool mv[kSortItems];
+    uint32_t rank[kSortItems];   // movers before this slot within the wave
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const int i = base + k * kWave + lane;
+        const bool valid = i < n;
+        kn[k] = valid ? key_new[i] : 0u;
+        mv[k] = valid && kn[k] != key_old[i];
+        const uint64_t bal = __ballot(mv[k]);
+        rank[k] = run + (uint32_t)__popcll(bal & lt);
+        run += (uint32_t)__popcll(bal);
+    }
+    if (lane == 0) wcnt[w] = run;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (int ww = 0; ww < w; ++ww) woff += wcnt[ww];
+    uint32_t block_movers = 0;
+    for (int ww = 0; ww < W; ++ww) block_movers += wcnt[ww];
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const int i = base + k * kWave + lane;
+        if (i >= n) continue;
+        const uint32_t mbefore = before + woff + rank[k];  // movers before slot i
+        if (mv[k]) {
+            mov_k[mbefore] = kn[k];
+            mov_v[mbefore] = (uint32_t)i;
+        } else {
+            const uint32_t sp = (uint32_t)i - mbefore;
+            stay_k[sp] = kn[k];
+            stay_v[sp] = (uint32_t)i;
+        }
+    }
+    if ((int)blockIdx.x == last_tile && tid == 0) {
+        const int movers = (int)(before + block_movers);
+        counts[0] = n - movers;
+        counts[1] = movers;
+    }
+}
+
+// (key, slot) order; slots of the two runs are distinct.
+__device__ __forceinline__ bool kv_less(uint32_t ka, uint32_t va, uint32_t kb, uint32_t vb)
+{
+    return ka < kb || (ka == kb && va < vb);
+}
+
+// Merge-path split of every output tile boundary: split[t] = stay elements
+// among the first min(t * kMergeTile, total) outputs.
+__global__ __launch_bounds__(256) void k_merge_partition(
+    const uint32_t *__restrict__ stay_k, const uint32_t *__restrict__ stay_v,
+    const uint32_t *__restrict__ mov_k, const uint32_t *__restrict__ mov_v,
+    const int *__restrict__ counts, int *__restrict__ split, int n_bounds)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n_bounds) return;
+    const int s = counts[0], m = counts[1];
+    const long long dd = (long long)t * kMergeTile;
+    const int d = (int)(dd < (long long)(s + m) ? dd : (long long)(s + m));
+    int lo = d - m > 0 ? d - m : 0, hi = d < s ? d : s;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int j = d - 1 - mid;
+        if (kv_less(stay_k[mid], stay_v[mid], mov_k[j], mov_v[j])) lo = mid + 1; else hi = mid;
+    }
+    split[t] = lo;
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_merge(
+    const uint32_t *__restrict__ stay_k, const uint32_t *__restrict__ stay_v,
+    const uint32_t *__restrict__ mov_k, const uint32_t *__restrict__ mov_v,
+    const int *__restrict__ counts, const int *__restrict__ split, uint32_t *__restrict__ out_k,
+    uint32_t *__restrict__ out_v)
+{
+    __shared__ uint32_t sk[kMergeTile];
+    __shared__ uint32_t sv[kMergeTile];
+    const int s = counts[0], m = counts[1], total = s + m;
+    const int d0 = blockIdx.x * kMergeTile;
+    if (d0 >= total) return;
+    const int d1 = min(d0 + kMergeTile, total);
+    const int i0 = split[blockIdx.x], i1 = split[blockIdx.x + 1];
+    const int j0 = d0 - i0, j1 = d1 - i1;
+    const int na = i1 - i0, nb = j1 - j0;
+    for (int q = threadIdx.x; q < na; q += kSortBlock) { sk[q] = stay_k[i0 + q]; sv[q] = stay_v[i0 + q]; }
+    for (int q = threadIdx.x; q < nb; q += kSortBlock) { sk[na + q] = mov_k[j0 + q]; sv[na + q] = mov_v[j0 + q]; }
+    __syncthreads();
+    // this thread's outputs: [diag, diag + kMergeItems) of the tile
+    const int diag = min((int)threadIdx.x * kMergeItems, na + nb);
+    int lo = diag - nb > 0 ? diag - nb : 0, hi = diag < na ? diag : na;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int j = na + diag - 1 - mid;
+        if (kv_less(sk[mid], sv[mid], sk[j], sv[j])) lo = mid + 1; else hi = mid;
+    }
+    int a = lo, b = na + diag - lo;
+    uint32_t rk[kMergeItems], rv[kMergeItems];
+#pragma unroll
+    for (int q = 0; q < kMergeItems; ++q) {
+        const bool take_a = a < na && (b >= na + nb || kv_less(sk[a], sv[a], sk[b], sv[b]));
+        const int src = take_a ? a : b;
+        rk[q] = src < na + nb ? sk[src] : 0u;
+        rv[q] = src < na + nb ? sv[src] : 0u;
+        a += take_a;
+        b += !take_a;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kMergeItems; ++q) {
+        const int o = (int)threadIdx.x * kMergeItems + q;
+        if (o < na + nb) { sk[o] = rk[q]; sv[o] = rv[q]; }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < na + nb; q += kSortBlock) {
+        out_k[d0 + q] = sk[q];
+        out_v[d0 + q] = sv[q];
+    }
+}
+
+size_t resort_scratch_bytes(int n_cap)
+{
+    const size_t n = (size_t)(n_cap > 0 ? n_cap : 1);
+    const size_t tiles = (size_t)div_up((long long)n, kSplitTile) + 1;
+    return 8 * n * sizeof(uint32_t) + (tiles + 1) * sizeof(uint32_t) + (tiles + 2) * sizeof(int) +
+           4 * sizeof(int) + 12 * 256;
+}
+
+void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem)
+{
+    auto align = [](char *p) { return (char *)(((uintptr_t)p + 255) & ~(uintptr_t)255); };
+    const size_t n = (size_t)(n_cap > 0 ? n_cap : 1);
+    r.n_cap = (int)n;
+    r.tiles = div_up((long long)n, kSplitTile);
+    char *p = align((char *)mem);
+    uint32_t **arrays[8] = {&r.stay_k, &r.stay_v, &r.mov_k, &r.mov_v,
+                            &r.movs_k, &r.movs_v, &r.alt_k, &r.alt_v};
+    for (auto a : arrays) { *a = (uint32_t *)p; p = align(p + n * sizeof(uint32_t)); }
+    r.tile_cnt = (uint32_t *)p; p = align(p + (size_t)(r.tiles + 1) * sizeof(uint32_t));
+    r.split = (int *)p; p = align(p + (size_t)(r.tiles + 2) * sizeof(int));
+    r.counts = (int *)p;
+}
+
+void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const int *n_dev,
+                        int key_bits, const ResortScratch &r, const SortScratch &s,
+                        uint32_t *out_k, uint32_t *out_v, hipStream_t stream, Profiler *prof)
+{
+    const double n = (double)r.n_cap;
+    DSMC_LAUNCH(prof, "resort_count", n * 8.0, k_split_count, r.tiles, kSortBlock, stream,
+                key_new, key_old, n_dev, r.tile_cnt);
+    DSMC_LAUNCH(prof, "resort_split", n * 16.0, k_split_scatter, r.tiles, kSortBlock, stream,
+                key_new, key_old, n_dev, r.tile_cnt, r.stay_k, r.stay_v, r.mov_k, r.mov_v,
+                r.counts);
+    radix_sort(r.mov_k, r.mov_v, r.alt_k, r.alt_v, r.movs_k, r.movs_v, r.counts + 1, key_bits, s,
+               stream, prof);
+    const int bounds = r.tiles + 1;  // merge tiles == split tiles (total <= n_cap)
+    DSMC_LAUNCH(prof, "resort_partition", (double)bounds * 64.0, k_merge_partition,
+                div_up(bounds, 256), 256, stream, r.stay_k, r.stay_v, r.movs_k, r.movs_v,
+                r.counts, r.split, bounds);
+    DSMC_LAUNCH(prof, "resort_merge", n * 16.0, k_merge, r.tiles, kSortBlock, stream, r.stay_k,
+                r.stay_v, r.movs_k, r.movs_v, r.counts, r.split, out_k, out_v);
 }
 
 // ---------------------------------------------------------------------------

@@ -49,6 +49,11 @@ enum { DSMC_FP64_PLAIN = 0, DSMC_FP32_KAHAN = 1 };
  * trials. */
 enum { DSMC_PAIR_UNIFORM = 0, DSMC_PAIR_REFERENCE = 1 };
 enum { DSMC_EVAP_SPIN = 1, DSMC_EVAP_ENERGY = 2 };      /* bit flags          */
+/* FP32 form: how each step's sort is done.  Both give the same permutation
+ * (the stable sort by cell key).  DSMC_SORT_INCREMENTAL re-sorts only the
+ * atoms whose cell changed since the previous sort and merges them into the
+ * rest; DSMC_SORT_FULL radix-sorts every key. */
+enum { DSMC_SORT_INCREMENTAL = 0, DSMC_SORT_FULL = 1 };
 
 /* One trap (one independent simulation).  Replaces the compile-time globals of
  * include/initialSystemParameters.cuh:14-33 and deviceSystemParameters.cuh:12-31
@@ -90,7 +95,8 @@ typedef struct dsmc_config {
     unsigned long long seed;/* Philox key                                         */
     int    block_size;      /* per-atom kernels; 0 = default                      */
     int    obs_capacity;    /* steps kept in the observables ring; 0 = default    */
-    int    reserved[6];
+    int    sort_mode;       /* DSMC_SORT_* (FP32 form)                            */
+    int    reserved[5];
 } dsmc_config;
 
 /* FP64 observables of one trap after one motion step. */

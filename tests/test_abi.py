@@ -95,6 +95,17 @@ def test_invalid_configuration_rejected(mutate, status, needle):
     assert needle in msg
 
 
+def test_invalid_sort_mode_rejected():
+    lib = _abi.load_library()
+    run = configs.quadrupole_1e4()
+    traps = (_abi.Trap * 1)(run.traps[0].to_c())
+    cfg = run.to_c()
+    cfg.sort_mode = 7
+    h = C.c_void_p()
+    assert lib.dsmc_create(C.byref(cfg), traps, 1, C.byref(h)) == _abi.DSMC_EINVAL
+    assert "sort_mode" in lib.dsmc_last_error(None).decode()
+
+
 def test_null_arguments_rejected():
     lib = _abi.load_library()
     h = C.c_void_p()

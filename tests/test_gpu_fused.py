@@ -121,6 +121,48 @@ def test_wave_collide_equals_serial(pair_mode, monkeypatch):
             np.testing.assert_array_equal(a, b)
 
 
+def test_incremental_sort_is_the_stable_sort():
+    """After a sorted move only the atoms that changed cell are re-sorted and
+    merged; the permutation must be numpy's stable argsort of the new keys."""
+    run = quad(200000, 40)
+    with Simulation(run) as sim:
+        sim.init_distribution()
+        sim.step(7)
+        for rep in range(3):
+            keys = sim.copy_out(_abi.FIELD_CELL_ID).astype(np.int64)
+            sim.sort_arrays()
+            perm = sim.copy_out(_abi.FIELD_PERM)
+            srt = sim.copy_out(_abi.FIELD_CELL_ID)
+            order = np.argsort(keys, kind="stable")[: len(perm)]
+            moved = int(np.count_nonzero(np.diff(keys) < 0))
+            print(f"re-sort {rep}: n={len(keys)}, descents={moved}", flush=True)
+            assert moved > 0
+            np.testing.assert_array_equal(perm, order)
+            np.testing.assert_array_equal(srt, keys[order])
+            sim.build_cell_table()
+            sim.collide()
+            sim.move_atoms()
+
+
+def test_incremental_sort_equals_full_sort():
+    """Whole batched runs (evaporation on, graph replay) with the incremental
+    and the full sort are bitwise identical."""
+    run = _batch_run()
+    out = {}
+    for mode in ("incremental", "full"):
+        print(f"batched run, sort mode {mode}", flush=True)
+        with Simulation(run.with_(sort_mode=mode)) as sim:
+            sim.init_distribution()
+            sim.step(41)
+            out[mode] = [sim.copy_out(f) for f in (_abi.FIELD_POS, _abi.FIELD_VEL,
+                                                   _abi.FIELD_ATOM_ID)]
+            out[mode].append(sim.obs_log()["collisions"].copy())
+            out[mode].append(sim.obs_log()["n_evap"].copy())
+    assert int(out["full"][4].sum()) > 0
+    for a, b in zip(out["incremental"], out["full"]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_graph_replay_equals_eager():
     run = quad(40000, 30)
     with Simulation(run) as a, Simulation(run.with_(use_graph=False)) as b:
