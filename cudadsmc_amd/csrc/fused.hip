@@ -144,16 +144,41 @@ __global__ __launch_bounds__(kFBlock) void k_table_fused(const uint32_t *__restr
         blk_traps[1] = last >= b0 ? last_le(kb, T + 1, (int)key[last]) : T;
     }
     __syncthreads();
-    const bool one_trap = blk_traps[0] == blk_traps[1];
+    if (blk_traps[0] == blk_traps[1]) {
+        // no trap boundary inside: only the range's first slot (a boundary
+        // when the previous slot belongs to another trap) and slot n
+        if (threadIdx.x == 0) {
+            const int i = b0;
+            const uint32_t k = (i < n) ? key[i] : tp.dead_key;
+            const int cur = (i == n) ? T : last_le(kb, T + 1, (int)k);
+            const int prev = (i == 0) ? -1 : last_le(kb, T + 1, (int)key[i - 1]);
+            for (int t = prev + 1; t <= cur; ++t) trap_start[t] = i;
+            if (cur != prev && prev < T && cur == T) *n_out_dev = i;
+        }
+        if (threadIdx.x == 1 && n >= b0 + 1 && n < b1) {
+            const int prev = last_le(kb, T + 1, (int)key[n - 1]);
+            for (int t = prev + 1; t <= T; ++t) trap_start[t] = n;
+            if (prev < T) *n_out_dev = n;
+        }
+        return;
+    }
+    // Several traps: every slot is examined.  All keys are loaded first
+    // (independent loads in flight together), then compared.
+    uint32_t kc[kTableItems], kp[kTableItems];
+#pragma unroll
     for (int it = 0; it < kTableItems; ++it) {
         const int i = b0 + it * kFBlock + threadIdx.x;
-        if (i >= b1) break;
-        if (one_trap && i != b0 && i != n) continue;
-        const uint32_t k = (i < n) ? key[i] : tp.dead_key;
-        const uint32_t kp = (i > 0) ? key[i - 1] : 0xFFFFFFFFu;
-        if (i > 0 && i < n && kp == k) continue;
+        kc[it] = (i < n) ? key[i] : tp.dead_key;
+        kp[it] = (i > 0 && i - 1 < n) ? key[i - 1] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int it = 0; it < kTableItems; ++it) {
+        const int i = b0 + it * kFBlock + threadIdx.x;
+        if (i >= b1) continue;
+        const uint32_t k = kc[it];
+        if (i > 0 && i < n && kp[it] == k) continue;
         const int cur = (i == n) ? T : last_le(kb, T + 1, (int)k);
-        const int prev = (i == 0) ? -1 : last_le(kb, T + 1, (int)kp);
+        const int prev = (i == 0) ? -1 : last_le(kb, T + 1, (int)kp[it]);
         if (cur != prev) {
             for (int t = prev + 1; t <= cur; ++t) trap_start[t] = i;
             if (prev < T && cur == T) *n_out_dev = i;

@@ -151,3 +151,37 @@ def test_init_distribution_matches_oracle(orc):
     st = orc.init_distribution(p, run.n_atoms, run.traps[0].T)
     np.testing.assert_allclose(pos, st["pos"], rtol=1e-13, atol=1e-20)
     np.testing.assert_allclose(vel, st["vel"], rtol=1e-13, atol=1e-18)
+
+
+def test_golden_index_fixture(orc):
+    """tests/golden/index_fixture.npz: 4096 positions of the 35^3 quadrupole
+    grid with atoms on cell faces, one ulp either side, on the grid edges,
+    outside the grid, NaN and inf; expected cell ids, permutation, cell table
+    and compaction survivors were written by the oracle."""
+    import ctypes as C
+    from pathlib import Path
+    from tests.gpu_util import DeviceArray
+
+    fx = np.load(Path(__file__).resolve().parent / "golden" / "index_fixture.npz")
+    base = configs.quadrupole_1e4()
+    trap = base.traps[0]
+    trap.n_atoms = len(fx["pos"])
+    run = base.with_(traps=[trap])
+    print("golden index fixture", flush=True)
+    with Simulation(run) as sim:
+        sim.set_state(fx["pos"], fx["vel"])
+        sim.index_atoms()
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_CELL_ID), fx["cell"])
+        sim.sort_arrays()
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_CELL_ID), fx["sorted_cell"])
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_ATOM_ID), fx["sorted_id"])
+        sim.build_cell_table()
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_CELL_START_END), fx["start_end"])
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_CELL_COUNT), fx["count"])
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_CELL_PREFIX), fx["prefix"])
+    lib = _abi.load_library()
+    with DeviceArray(lib, fx["sorted_id"]) as vals, DeviceArray(lib, fx["stencil"]) as sten:
+        n_out = C.c_int()
+        assert lib.dsmc_remove_if(vals.ptr, sten.ptr, len(fx["sorted_id"]), C.byref(n_out), None) == 0
+        got = vals.get()[: n_out.value]
+    np.testing.assert_array_equal(got, fx["survivors"])
