@@ -214,7 +214,7 @@ def cpu_baseline(target_s=12.0):
     sv = np.full(orc.ncells(p) + 1, orc.sigvrmax_init(p, t.T))
     probe = orc.run(p, st, 50, 0, sv)
     per_step = probe["seconds"] / 50
-    n = int(max(50, min(5000, target_s / max(per_step, 1e-6))))
+    n = int(max(50, min(200000, target_s / max(per_step, 1e-6))))
     res = orc.run(p, st, n, 50, sv)
     return dict(value=res["atom_steps"] / res["seconds"], unit="atom-steps/s",
                 cores=res["threads"], kind="port",
@@ -235,6 +235,8 @@ def main():
                     help="only run the streaming-copy bandwidth calibration")
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the rocprofv3 counter runs (roofline.traffic = null)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the step's kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
 
@@ -262,6 +264,8 @@ def main():
 
     batched = configs.batched_traps(steps=args.steps)
     batched.obs_capacity = args.steps + args.warmup + args.profile_steps + 64
+    if args.no_graph:
+        batched.use_graph = False
     roof = None
     kernels = None
     if "batched" in legs:

@@ -218,6 +218,8 @@ static TrapDev make_trapdev(const dsmc_config &c, const dsmc_trap &t, int index,
     d.mu = t.mu;
     d.neg_mu_over_m = (-t.mu) / c.mass;
     d.alpha = t.n_real / (double)t.n_atoms;
+    d.coll_c1 = d.alpha * c.lpc * c.dt / d.cell_vol;
+    d.coll_c1s = d.coll_c1 * c.sigma;
     double Tm = t.T;
     double Tax[3];
     for (int k = 0; k < 3; ++k) Tax[k] = t.T_axis[k] > 0.0 ? t.T_axis[k] : t.T;

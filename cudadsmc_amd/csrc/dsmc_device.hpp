@@ -42,6 +42,8 @@ struct TrapDev {
     double mu;             // J/T
     double neg_mu_over_m;  // (-mu) / m          (moveAtoms.cu:102)
     double alpha;          // real atoms per test particle
+    double coll_c1;        // alpha lpc dt / cell_vol (collision count per pair and sigma v_r)
+    double coll_c1s;       // coll_c1 * sigma
     double sigvrmax0;      // initSigvrmax value (setUp.cu:289)
     double kT_cut;         // kB * T_cut (energy cut); <= 0 disables
     double eta_start, eta_end;

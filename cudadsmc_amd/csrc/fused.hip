@@ -237,14 +237,23 @@ __device__ __forceinline__ bool cell_setup(const TrapDev *__restrict__ traps, co
     const TrapDev &tr = traps[c.t];
     c.cell = c.k - (uint32_t)tr.key_base;
     if ((int)c.cell >= tr.n_cells) return false;  // overflow cell: no collisions
-    const double lpc = cp.lpc;
     c.smax = sigvrmax[c.k];
     const double Mc = 0.5 * (c.n - 1) * c.n;
-    const double X = Mc * tr.alpha * lpc * cp.dt * c.smax / tr.cell_vol;
-    const double lambda = ceil(X) / Mc;
-    // reference pair mode keeps collisions.cu:322's truncation of Mc * lambda
-    c.ncol = cp.pair_mode ? (int)(Mc * lambda) : (int)ceil(X);
-    c.pc = (float)(tr.alpha * lpc * cp.dt / tr.cell_vol * cp.sigma / lambda);
+    if (cp.pair_mode) {
+        // reference pair mode: collisions.cu:318-322 including the truncation
+        // of Mc * lambda
+        const double lpc = cp.lpc;
+        const double X = Mc * tr.alpha * lpc * cp.dt * c.smax / tr.cell_vol;
+        const double lambda = ceil(X) / Mc;
+        c.ncol = (int)(Mc * lambda);
+        c.pc = (float)(tr.alpha * lpc * cp.dt / tr.cell_vol * cp.sigma / lambda);
+    } else {
+        // ceil(X) trials, acceptance alpha lpc dt sigma v_r / (Vc lambda) with
+        // lambda = ceil(X) / Mc; the per-trap factor is precomputed
+        const double cX = ceil(Mc * tr.coll_c1 * c.smax);
+        c.ncol = (int)cX;
+        c.pc = (float)(tr.coll_c1s * Mc / cX);
+    }
     return c.ncol > 0;
 }
 
