@@ -120,14 +120,17 @@ struct dsmc_ctx {
     int obs_cap = 0;
     int block = 256;
     bool have_sort = false;         // FP32: perm valid for the next move
-    bool srt_valid = false;         // FP32: key_b[i] is the key of slot i of the current
-                                    // state (its slots are in sorted order of key_b)
+    bool srt_valid = false;         // FP32: srt_k[srt_cur][i] is the key of slot i of the
+                                    // current state (slots in sorted order of those keys)
     std::vector<void *> allocs;
 
     // common device buffers
     TrapDev *d_traps = nullptr;
     int *d_key_base = nullptr;
     uint32_t *key_a = nullptr, *key_b = nullptr, *val_a = nullptr, *val_b = nullptr;
+    uint32_t *srt_k[2] = {nullptr, nullptr}, *srt_v[2] = {nullptr, nullptr};  // FP32 sort outputs
+    int srt_cur = 0;                // FP32: srt_*[srt_cur] holds the last sort's result
+    int graph_srt = 0;              // srt_cur at graph capture
     const uint32_t *sorted_key = nullptr, *perm = nullptr;
     SortScratch sort;
     ResortScratch res;              // FP32: incremental re-sort
@@ -149,6 +152,7 @@ struct dsmc_ctx {
     // FP32 form
     FusedPtrs fs[2];
     uint32_t *cell_start = nullptr, *cell_end = nullptr;
+    uint32_t *blk_moved = nullptr, *blk_dead = nullptr;   // per move block, for the re-sort
     int *trap_start = nullptr;
     double *acc = nullptr, *acc2 = nullptr;
     ObsRecords rec{};
@@ -373,6 +377,12 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
             x->fs[b].id = x->alloc<int>(N);
         }
         resort_scratch_bind(x->res, N, x->alloc<char>(resort_scratch_bytes(N), false));
+        // one entry per block of the smallest move block size, whole 2048-slot
+        // tiles (never-written entries stay zero)
+        x->blk_moved = x->alloc<uint32_t>((size_t)div_up(N, 2048) * (2048 / 64) + 64);
+        x->blk_dead = x->alloc<uint32_t>((size_t)div_up(N, 2048) * (2048 / 64) + 64);
+        x->srt_k[0] = x->key_b; x->srt_v[0] = x->val_b;
+        x->srt_k[1] = x->alloc<uint32_t>(N); x->srt_v[1] = x->alloc<uint32_t>(N);
         x->cell_start = x->alloc<uint32_t>(x->n_keys, false);
         x->cell_end = x->alloc<uint32_t>(x->n_keys, false);
         x->trap_start = x->alloc<int>(n_traps + 1, false);
@@ -535,6 +545,9 @@ static MoveParams move_params(const dsmc_ctx *x)
     mp.n_traps = x->T;
     mp.obs_cap = x->obs_cap;
     mp.dead_key = x->dead_key;
+    mp.key_old = nullptr;
+    mp.blk_moved = x->blk_moved;
+    mp.blk_dead = x->blk_dead;
     return mp;
 }
 
@@ -544,21 +557,23 @@ static void fused_index(dsmc_ctx *x)
                        x->n_cap, x->stream, &x->prof);
 }
 
-// Stable sort of the keys in key_a into (key_b, val_b).  When the current
-// state is in sorted order of key_b (after a sorted move), only the atoms
-// whose key changed are sorted and merged into the rest; the result is the
-// same permutation.
+// Stable sort of the keys in key_a into the other sort buffer.  When the
+// current state is in sorted order of the previous sort's keys (after a
+// sorted move), only the atoms whose key changed are placed among the rest;
+// the result is the same permutation.
 static void fused_sort(dsmc_ctx *x)
 {
     const int *n = x->n_dev + x->cur;
-    if (x->srt_valid && x->cfg.sort_mode == DSMC_SORT_INCREMENTAL)
-        resort_incremental(x->key_a, x->key_b, n, x->key_bits, x->res, x->sort, x->key_b,
-                           x->val_b, x->stream, &x->prof);
+    uint32_t *ok = x->srt_k[x->srt_cur ^ 1], *ov = x->srt_v[x->srt_cur ^ 1];
+    if (x->srt_valid && x->cfg.sort_mode == DSMC_SORT_INCREMENTAL && resort_supported(x->n_cap))
+        resort_incremental(x->key_a, x->srt_k[x->srt_cur], n, x->dead_key, x->blk_moved,
+                           x->blk_dead, x->block, x->res, ok, ov, x->stream, &x->prof);
     else
-        radix_sort(x->key_a, nullptr, x->res.alt_k, x->res.alt_v, x->key_b, x->val_b, n,
-                   x->key_bits, x->sort, x->stream, &x->prof);
-    x->sorted_key = x->key_b;
-    x->perm = x->val_b;
+        radix_sort(x->key_a, nullptr, x->res.alt_k, x->res.alt_v, ok, ov, n, x->key_bits,
+                   x->sort, x->stream, &x->prof);
+    x->srt_cur ^= 1;
+    x->sorted_key = ok;
+    x->perm = ov;
     x->have_sort = true;
     x->srt_valid = false;   // the state is not yet in the new order
 }
@@ -583,7 +598,11 @@ static void fused_move(dsmc_ctx *x)
 {
     // after a sort the alive count of the sorted order is n_dev[cur ^ 1]
     const int *n = x->have_sort ? x->n_dev + (x->cur ^ 1) : x->n_dev + x->cur;
-    launch_move_fused(x->block, x->d_traps, move_params(x), x->fs[x->cur], x->fs[x->cur ^ 1],
+    MoveParams mp = move_params(x);
+    // a sorted move also counts, per move block, the atoms whose key changed:
+    // the input of the next step's incremental re-sort
+    if (x->have_sort) mp.key_old = x->sorted_key;
+    launch_move_fused(x->block, x->d_traps, mp, x->fs[x->cur], x->fs[x->cur ^ 1],
                       x->have_sort ? x->perm : nullptr, x->trap_start, n, x->key_a,
                       x->evap_pos_f, x->evap_vel_f, x->rec, x->step_dev, x->n_cap, x->stream,
                       &x->prof);
@@ -593,9 +612,10 @@ static void fused_move(dsmc_ctx *x)
     if (!x->have_sort)  // keep the count with the state it describes
         DSMC_HIP(hipMemcpyAsync(x->n_dev + (x->cur ^ 1), x->n_dev + x->cur, sizeof(int),
                                 hipMemcpyDeviceToDevice, x->stream));
-    // a sorted move leaves the state in sorted order of key_b; an unsorted one
-    // keeps the slot order (and so the validity of key_b)
-    if (x->have_sort) x->srt_valid = true;
+    // a sorted move leaves the state in sorted order of the last sort's keys
+    // and has counted the changed keys; after an unsorted move the next sort
+    // is a full one
+    x->srt_valid = x->have_sort;
     x->have_sort = false;
     x->cur ^= 1;
 }
@@ -614,8 +634,16 @@ static bool graph_ok(const dsmc_ctx *x) { return x->fp32 && x->cfg.use_graph && 
 
 static void fused_graph_steps(dsmc_ctx *x, int pairs)
 {
+    if (x->graph_exec && x->graph_srt != x->srt_cur) {
+        // captured with the other sort buffer holding the previous sort
+        (void)hipGraphExecDestroy(x->graph_exec);
+        (void)hipGraphDestroy(x->graph);
+        x->graph_exec = nullptr;
+        x->graph = nullptr;
+    }
     if (!x->graph_exec) {
         const int cur0 = x->cur;
+        x->graph_srt = x->srt_cur;
         const long long step0 = x->step, cs0 = x->coll_steps;
         DSMC_HIP(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
         try {
@@ -630,6 +658,7 @@ static void fused_graph_steps(dsmc_ctx *x, int pairs)
         DSMC_HIP(hipStreamEndCapture(x->stream, &x->graph));
         DSMC_HIP(hipGraphInstantiate(&x->graph_exec, x->graph, nullptr, nullptr, 0));
         x->cur = cur0;
+        x->srt_cur = x->graph_srt;
         x->step = step0;
         x->coll_steps = cs0;
     }
@@ -1326,11 +1355,15 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
                             hipMemcpyDeviceToDevice, ctx->stream));
     DSMC_HIP(hipMemsetAsync(ctx->acc, 0, sizeof(double) * kAccK * ctx->T, ctx->stream));
     DSMC_HIP(hipStreamSynchronize(ctx->stream));
-    if (ctx->block != best_bs && ctx->graph_exec) {
-        (void)hipGraphExecDestroy(ctx->graph_exec);
-        (void)hipGraphDestroy(ctx->graph);
-        ctx->graph_exec = nullptr;
-        ctx->graph = nullptr;
+    if (ctx->block != best_bs) {
+        if (ctx->graph_exec) {
+            (void)hipGraphExecDestroy(ctx->graph_exec);
+            (void)hipGraphDestroy(ctx->graph);
+            ctx->graph_exec = nullptr;
+            ctx->graph = nullptr;
+        }
+        // the per-block counts of the last move are for the old block size
+        ctx->srt_valid = false;
     }
     ctx->block = best_bs;
     if (block_size_out) *block_size_out = best_bs;

@@ -322,6 +322,12 @@ __device__ __forceinline__ F3 accel_f(const TrapDev &t, float x, float y, float 
     return a;
 }
 
+// Streaming stores for large outputs: marked non-temporal so that they do not
+// stay dirty in the (per-XCD, non-coherent) L2, whose write-back at the end
+// of a kernel would otherwise be paid before the next kernel can start.
+template <typename T>
+__device__ __forceinline__ void st_stream(T *p, T v) { __builtin_nontemporal_store(v, p); }
+
 // ---------------------------------------------------------------------------
 // Wave64 helpers.
 // ---------------------------------------------------------------------------
@@ -405,6 +411,20 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double *tmp)
 }
 
 // Mask of the lanes whose 8-bit digit equals this lane's, among `active`.
+// Lanes of `active` whose low B bits of v equal this lane's.
+template <int B>
+__device__ __forceinline__ uint64_t match_bits(uint32_t v, uint64_t active)
+{
+    uint64_t m = active;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const bool bit = (v >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        m &= bit ? bal : ~bal;
+    }
+    return m;
+}
+
 __device__ __forceinline__ uint64_t match_digit8(uint32_t d, uint64_t active)
 {
     uint64_t m = active;

@@ -90,20 +90,26 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
                 hipStream_t stream, Profiler *prof);
 
 // Incremental re-sort (see primitives.hip): the stable sort of key_new[0..n)
-// by (key, slot), for slots that are in stable sorted order of key_old.
+// by (key, slot), for slots that are in stable sorted order of key_old, into
+// out (which must not alias key_old).  dead_key sorts last.
 struct ResortScratch {
     int n_cap = 0, tiles = 0;
     uint32_t *stay_k = nullptr, *stay_v = nullptr, *mov_k = nullptr, *mov_v = nullptr;
-    uint32_t *movs_k = nullptr, *movs_v = nullptr, *alt_k = nullptr, *alt_v = nullptr;
-    uint32_t *tile_cnt = nullptr;
-    int *split = nullptr;
-    int *counts = nullptr;       // [0] stay, [1] movers
+    uint32_t *mov_q = nullptr, *mov_r = nullptr, *bk_k = nullptr, *bk_v = nullptr, *bk_q = nullptr;
+    uint32_t *srt_k = nullptr, *srt_v = nullptr, *srt_q = nullptr;
+    uint32_t *alt_k = nullptr, *alt_v = nullptr, *spare = nullptr;
+    uint32_t *tile_cnt = nullptr, *tile_dead = nullptr, *bcnt = nullptr, *bex = nullptr;
+    int *counts = nullptr;       // [0] stays, [1] movers, [2] and [3] removed
 };
 size_t resort_scratch_bytes(int n_cap);
 void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem);
+bool resort_supported(int n_cap);
+// blk_moved / blk_dead: per block of move_block slots, the atoms whose key
+// changed (not removed) and the removed atoms (written by the fused move).
 void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const int *n_dev,
-                        int key_bits, const ResortScratch &r, const SortScratch &s,
-                        uint32_t *out_k, uint32_t *out_v, hipStream_t stream, Profiler *prof);
+                        uint32_t dead_key, const uint32_t *blk_moved, const uint32_t *blk_dead,
+                        int move_block, const ResortScratch &r, uint32_t *out_k, uint32_t *out_v,
+                        hipStream_t stream, Profiler *prof);
 
 // Exclusive scan of n ints (n known on the host); partial holds div_up(n, tile).
 void exclusive_scan(const int *in, int *out, int n, int *partial, hipStream_t stream,

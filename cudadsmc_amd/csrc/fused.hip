@@ -41,6 +41,16 @@ __device__ __forceinline__ int last_le(const int *arr, int n, int v)
     return lo;
 }
 
+// Largest t in [lo, hi] with arr[t] <= v, given arr[lo] <= v.
+__device__ __forceinline__ int last_le_in(const int *arr, int lo, int hi, int v)
+{
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (arr[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ void load_table(int *lds, const int *src, int cnt)
 {
     for (int j = threadIdx.x; j < cnt; j += blockDim.x) lds[j] = src[j];
@@ -163,7 +173,9 @@ __global__ __launch_bounds__(kFBlock) void k_table_fused(const uint32_t *__restr
         return;
     }
     // Several traps: every slot is examined.  All keys are loaded first
-    // (independent loads in flight together), then compared.
+    // (independent loads in flight together), then compared; the trap of a
+    // key inside the range is searched among the range's own traps only.
+    const int t0 = blk_traps[0], t1 = blk_traps[1];
     uint32_t kc[kTableItems], kp[kTableItems];
 #pragma unroll
     for (int it = 0; it < kTableItems; ++it) {
@@ -177,8 +189,10 @@ __global__ __launch_bounds__(kFBlock) void k_table_fused(const uint32_t *__restr
         if (i >= b1) continue;
         const uint32_t k = kc[it];
         if (i > 0 && i < n && kp[it] == k) continue;
-        const int cur = (i == n) ? T : last_le(kb, T + 1, (int)k);
-        const int prev = (i == 0) ? -1 : last_le(kb, T + 1, (int)kp[it]);
+        const int cur = (i == n) ? T : last_le_in(kb, t0, t1, (int)k);
+        const int prev = (i == 0) ? -1
+                         : (i == b0) ? last_le(kb, T + 1, (int)kp[it])
+                                     : last_le_in(kb, t0, t1, (int)kp[it]);
         if (cur != prev) {
             for (int t = prev + 1; t <= cur; ++t) trap_start[t] = i;
             if (prev < T && cur == T) *n_out_dev = i;
@@ -501,6 +515,7 @@ __global__ __launch_bounds__(BS) void k_move_fused(
     double vals[4] = { 0.0, 0.0, 0.0, 0.0 };
     double lost_e = 0.0;
     bool lost = false;
+    bool moved = false;
     int t = -1;
     if (i < n) {
         t = t_first;
@@ -586,14 +601,15 @@ __global__ __launch_bounds__(BS) void k_move_fused(
             kout = (uint32_t)(tr.key_base + cell_of_fast(tr, X, Y, Z));
             vals[0] = VX * VX; vals[1] = VY * VY; vals[2] = VZ * VZ;
             vals[3] = ep;
+            if (mp.key_old) moved = kout != mp.key_old[i];
         }
-        B.p[0][i] = x; B.p[1][i] = y; B.p[2][i] = z;
-        B.cp[0][i] = cx; B.cp[1][i] = cy; B.cp[2][i] = cz;
-        B.v[0][i] = vx; B.v[1][i] = vy; B.v[2][i] = vz;
-        B.cv[0][i] = cvx; B.cv[1][i] = cvy; B.cv[2][i] = cvz;
-        if (mp.spin) { B.s[0][i] = sx; B.s[1][i] = sy; B.s[2][i] = sz; }
-        B.id[i] = id;
-        key_out[i] = kout;
+        st_stream(&B.p[0][i], x); st_stream(&B.p[1][i], y); st_stream(&B.p[2][i], z);
+        st_stream(&B.cp[0][i], cx); st_stream(&B.cp[1][i], cy); st_stream(&B.cp[2][i], cz);
+        st_stream(&B.v[0][i], vx); st_stream(&B.v[1][i], vy); st_stream(&B.v[2][i], vz);
+        st_stream(&B.cv[0][i], cvx); st_stream(&B.cv[1][i], cvy); st_stream(&B.cv[2][i], cvz);
+        if (mp.spin) { st_stream(&B.s[0][i], sx); st_stream(&B.s[1][i], sy); st_stream(&B.s[2][i], sz); }
+        st_stream(&B.id[i], id);
+        st_stream(&key_out[i], kout);
     }
 
     // Per-trap block sums.  Atoms are trap-major, so a block covers traps
@@ -602,7 +618,21 @@ __global__ __launch_bounds__(BS) void k_move_fused(
     // later traps go to acc_over by atomics.  Only the block holding a trap's
     // first slot can do that, so each accumulator receives at most one term
     // and the totals do not depend on the order of execution.
+    __shared__ uint32_t wcnt[BS / kWave][2];
+    if (mp.key_old) {   // per-wave counts, published by the barrier below
+        const uint64_t bm = __ballot(moved), bd = __ballot(lost);
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            wcnt[threadIdx.x / kWave][0] = (uint32_t)__popcll(bm);
+            wcnt[threadIdx.x / kWave][1] = (uint32_t)__popcll(bd);
+        }
+    }
     const bool any_lost = __syncthreads_or(lost);
+    if (mp.key_old && threadIdx.x == 0) {
+        uint32_t nm = 0, nd = 0;
+        for (int w = 0; w < BS / kWave; ++w) { nm += wcnt[w][0]; nd += wcnt[w][1]; }
+        mp.blk_moved[blockIdx.x] = nm;
+        mp.blk_dead[blockIdx.x] = nd;
+    }
     if (b0 < n) {
         const int nseg = t_last - t_first + 1;
         for (int sg = 0; sg < nseg; ++sg) {
@@ -661,29 +691,36 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
         }
     }
     block_sum<256, kAccK>(a, red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < kAccK; ++q)
-            __hip_atomic_store(&lvl2[((size_t)t * P + p) * kAccK + q], a[q], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        last = atomicAdd(&cnt[t], 1u) == (unsigned)(P - 1);
-    }
-    __syncthreads();
-    if (!last || threadIdx.x != 0) return;
-    __threadfence();
     double f[kAccK];
+    if (P == 1) {
+        // one block per trap: its sum is the trap's total
+        if (threadIdx.x != 0) return;
 #pragma unroll
-    for (int q = 0; q < kAccK; ++q) f[q] = 0.0;
-    for (int pp = 0; pp < P; ++pp)
-        for (int q = 0; q < kAccK; ++q)
-            f[q] += __hip_atomic_load(&lvl2[((size_t)t * P + pp) * kAccK + q], __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 0; q < kAccK; ++q) f[q] = a[q];
+    } else {
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int q = 0; q < kAccK; ++q)
+                __hip_atomic_store(&lvl2[((size_t)t * P + p) * kAccK + q], a[q], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            last = atomicAdd(&cnt[t], 1u) == (unsigned)(P - 1);
+        }
+        __syncthreads();
+        if (!last || threadIdx.x != 0) return;
+        __threadfence();
+#pragma unroll
+        for (int q = 0; q < kAccK; ++q) f[q] = 0.0;
+        for (int pp = 0; pp < P; ++pp)
+            for (int q = 0; q < kAccK; ++q)
+                f[q] += __hip_atomic_load(&lvl2[((size_t)t * P + pp) * kAccK + q], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+        cnt[t] = 0u;
+    }
     for (int q = 0; q < kAccK; ++q) {
         f[q] += acc_over[t * kAccK + q];
         acc_over[t * kAccK + q] = 0.0;
     }
-    cnt[t] = 0u;
     const unsigned long long c = coll_accum[t];
     coll_accum[t] = 0ull;
     dsmc_observables &o = obs_log[(step % obs_cap) * T + t];

@@ -38,6 +38,12 @@ struct MoveParams {
     int    n_traps;
     int    obs_cap;      // records in the observables ring (per trap)
     uint32_t dead_key;
+    // when key_old is set: per move block, the atoms whose new key differs
+    // from key_old (the previous sort's key of the slot) and the removed atoms,
+    // for the incremental re-sort of the next step
+    const uint32_t *key_old;
+    uint32_t *blk_moved;
+    uint32_t *blk_dead;
 };
 
 struct TableParams {

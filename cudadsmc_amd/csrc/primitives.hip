@@ -199,187 +199,407 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 // ---------------------------------------------------------------------------
 // Incremental re-sort.  Input: keys key_new[0..n) of slots that were in
 // stable sorted order of key_old[0..n) (the previous sort's keys).  Output:
-// the stable sort of key_new by (key, slot) -- identical to a full radix sort
-// -- built as
-//   1. split: slots whose key did not change ("stay", already in order) and
-//      the others ("movers"), both in slot order;
-//   2. radix sort of the movers alone (stable, so ties stay in slot order);
-//   3. merge path of the two sorted runs by (key, slot).
-// Between time steps only atoms that crossed a cell face (a few percent)
-// are movers, so the full-width sort runs on a few percent of the data.
+// the stable sort of key_new by (key, slot) -- the same permutation a full
+// radix sort gives -- in a buffer other than key_old.  Between time steps
+// only the atoms that crossed a cell face change key (a few percent), so
+// instead of sorting:
+//   1. split: slots whose key did not change ("stays": already in (key, slot)
+//      order), live "movers" and removed atoms (dead key, which sorts last:
+//      written straight to the end of the output, in slot order);
+//   2. locate: each mover's insertion point q = number of stays that precede
+//      it (binary search); movers are bucketed by the 2048-stay tile of q;
+//   3. place: movers are copied into their bucket;
+//   4. merge: one block per stay tile ranks its bucket's movers by counting
+//      (buckets hold a few dozen), and writes stays and movers to their final
+//      positions: stay g goes to g + (movers before it), a mover of rank r in
+//      the tile's bucket to q + (movers in earlier buckets) + r.
+// No kernel waits on another block.
 // ---------------------------------------------------------------------------
-constexpr int kSplitTile = kSortTile;    // slots per split block (256 x 8)
-constexpr int kMergeTile = kSortTile;    // outputs per merge block
-constexpr int kMergeItems = kMergeTile / kSortBlock;
+constexpr int kSplitTile = kSortTile;          // slots per split block (256 x 8)
+constexpr int kStayTile = 2048;                // stays per bucket
+constexpr int kStayShift = 11;
+constexpr int kBucketLds = 1024;               // movers a bucket ranks in LDS
+constexpr int kBucketBits = 13;
+constexpr int kMaxBuckets = 1 << kBucketBits;  // LDS bucket table of the place kernel
+static_assert((1 << kStayShift) == kStayTile, "stay tile");
 
-__global__ __launch_bounds__(kSortBlock) void k_split_count(const uint32_t *__restrict__ key_new,
-                                                            const uint32_t *__restrict__ key_old,
-                                                            const int *__restrict__ n_dev,
-                                                            uint32_t *__restrict__ tile_cnt)
+// Per split tile: movers and removed atoms before it (exclusive prefixes of
+// the counts the move kernel left per move block) and the removed total.
+// Block k handles tiles [256 k, 256 k + 256) and adds up the earlier tiles
+// itself (no block waits for another).
+constexpr int kTilesBlock = 256;
+
+__global__ __launch_bounds__(kTilesBlock) void k_ins_tiles(const uint32_t *__restrict__ blk_moved,
+                                                           const uint32_t *__restrict__ blk_dead,
+                                                           int per_tile, const int *__restrict__ n_dev,
+                                                           uint32_t *__restrict__ pre_m,
+                                                           uint32_t *__restrict__ pre_d,
+                                                           int *__restrict__ counts)
 {
-    __shared__ uint32_t tmp[kSortBlock / kWave + 1];
+    __shared__ uint32_t tmp[kTilesBlock / kWave + 1];
     const int n = *n_dev;
-    const int tile0 = blockIdx.x * kSplitTile;
-    uint32_t c = 0;
-    if (tile0 < n) {
+    const int tiles = n > 0 ? (n - 1) / kSplitTile + 1 : 1;
+    const int t0 = blockIdx.x * kTilesBlock;
+    if (t0 >= tiles) return;   // whole block leaves together
+    // earlier tiles: their move blocks [0, t0 * per_tile), a multiple of 1024
+    // entries read as 16-byte vectors, four per thread in flight
+    uint32_t bm = 0, bd = 0;
+    {
+        const uint4 *vm = (const uint4 *)blk_moved, *vd = (const uint4 *)blk_dead;
+        const int nv = t0 * per_tile / 4;
+        for (int v0 = 0; v0 < nv; v0 += 4 * kTilesBlock) {
+            uint4 a[4], d[4];
 #pragma unroll
-        for (int k = 0; k < kSortItems; ++k) {
-            const int i = tile0 + k * kSortBlock + threadIdx.x;
-            if (i < n) c += key_new[i] != key_old[i];
+            for (int u = 0; u < 4; ++u) {
+                const int v = v0 + u * kTilesBlock + (int)threadIdx.x;
+                a[u] = v < nv ? vm[v] : make_uint4(0, 0, 0, 0);
+                d[u] = v < nv ? vd[v] : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                bm += a[u].x + a[u].y + a[u].z + a[u].w;
+                bd += d[u].x + d[u].y + d[u].z + d[u].w;
+            }
         }
     }
-    uint32_t total;
-    (void)block_exclusive_scan<kSortBlock>(c, tmp, &total);
-    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = total;
+    uint32_t tot;
+    (void)block_exclusive_scan<kTilesBlock>(bm, tmp, &tot);
+    bm = tot;
+    (void)block_exclusive_scan<kTilesBlock>(bd, tmp, &tot);
+    bd = tot;
+    const int t = t0 + (int)threadIdx.x;
+    uint32_t sm = 0, sd = 0;
+    if (t < tiles) {
+#pragma unroll 8
+        for (int b = t * per_tile; b < (t + 1) * per_tile; ++b) { sm += blk_moved[b]; sd += blk_dead[b]; }
+    }
+    uint32_t tm, td;
+    const uint32_t em = block_exclusive_scan<kTilesBlock>(sm, tmp, &tm);
+    const uint32_t ed = block_exclusive_scan<kTilesBlock>(sd, tmp, &td);
+    if (t < tiles) {
+        pre_m[t] = bm + em;
+        pre_d[t] = bd + ed;
+    }
+    if (t0 + kTilesBlock >= tiles && threadIdx.x == 0) counts[3] = (int)(bd + td);
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_split_scatter(
+__global__ __launch_bounds__(kSortBlock) void k_ins_split(
     const uint32_t *__restrict__ key_new, const uint32_t *__restrict__ key_old,
-    const int *__restrict__ n_dev, const uint32_t *__restrict__ tile_cnt,
+    const int *__restrict__ n_dev, uint32_t dead_key, int tiles,
+    const uint32_t *__restrict__ pre_m, const uint32_t *__restrict__ pre_d,
     uint32_t *__restrict__ stay_k, uint32_t *__restrict__ stay_v, uint32_t *__restrict__ mov_k,
-    uint32_t *__restrict__ mov_v, int *__restrict__ counts)
+    uint32_t *__restrict__ mov_v, uint32_t *__restrict__ mov_p, uint32_t *__restrict__ out_k,
+    uint32_t *__restrict__ out_v, int *__restrict__ counts, uint32_t *__restrict__ bcnt)
 {
     constexpr int W = kSortBlock / kWave;
-    __shared__ uint32_t tmp[W + 1];
-    __shared__ uint32_t wcnt[W];
+    __shared__ uint32_t wcnt[2][W];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    // clear the bucket counts for the locate step (tiles + 2 entries)
+    if (tid == 0) bcnt[blockIdx.x] = 0u;
+    if (blockIdx.x == 0 && tid < 2) bcnt[tiles + tid] = 0u;
     const int n = *n_dev;
     const int tile0 = blockIdx.x * kSplitTile;
     const int last_tile = n > 0 ? (n - 1) / kSplitTile : 0;
     if ((int)blockIdx.x > last_tile) return;  // whole block leaves together
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
-    // movers in earlier tiles (each block sums the counts itself: no waiting)
-    uint32_t before = 0;
-    for (int t = tid; t < (int)blockIdx.x; t += kSortBlock) before += tile_cnt[t];
-    uint32_t tot;
-    (void)block_exclusive_scan<kSortBlock>(before, tmp, &tot);
-    before = tot;
+    // movers and removed atoms in earlier tiles, removed atoms in all tiles
+    const uint32_t mb = pre_m[blockIdx.x], db = pre_d[blockIdx.x], dall = (uint32_t)counts[3];
     // each wave owns a contiguous run of kWave * kSortItems slots
     const int base = tile0 + w * (kWave * kSortItems);
     const uint64_t lt = lanemask_lt();
     uint32_t kn[kSortItems];
-    bool mv[kSortItems];
-    uint32_t rank[kSortItems];   // movers before this slot within the wave
-    uint32_t run = 0;
+    uint8_t kind[kSortItems];    // 0 stay, 1 mover, 2 removed
+    uint32_t rm[kSortItems], rd[kSortItems];
+    uint32_t runm = 0, rund = 0;
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
         const int i = base + k * kWave + lane;
         const bool valid = i < n;
         kn[k] = valid ? key_new[i] : 0u;
-        mv[k] = valid && kn[k] != key_old[i];
-        const uint64_t bal = __ballot(mv[k]);
-        rank[k] = run + (uint32_t)__popcll(bal & lt);
-        run += (uint32_t)__popcll(bal);
+        const bool dead = valid && kn[k] == dead_key;
+        const bool mov = valid && !dead && kn[k] != key_old[i];
+        kind[k] = dead ? 2 : (mov ? 1 : 0);
+        const uint64_t bm = __ballot(mov), bd = __ballot(dead);
+        rm[k] = runm + (uint32_t)__popcll(bm & lt);
+        rd[k] = rund + (uint32_t)__popcll(bd & lt);
+        runm += (uint32_t)__popcll(bm);
+        rund += (uint32_t)__popcll(bd);
     }
-    if (lane == 0) wcnt[w] = run;
+    if (lane == 0) { wcnt[0][w] = runm; wcnt[1][w] = rund; }
     __syncthreads();
-    uint32_t woff = 0;
-    for (int ww = 0; ww < w; ++ww) woff += wcnt[ww];
-    uint32_t block_movers = 0;
-    for (int ww = 0; ww < W; ++ww) block_movers += wcnt[ww];
+    uint32_t wm = 0, wd = 0, bm_all = 0;
+    for (int ww = 0; ww < W; ++ww) {
+        if (ww < w) { wm += wcnt[0][ww]; wd += wcnt[1][ww]; }
+        bm_all += wcnt[0][ww];
+    }
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
         const int i = base + k * kWave + lane;
         if (i >= n) continue;
-        const uint32_t mbefore = before + woff + rank[k];  // movers before slot i
-        if (mv[k]) {
+        const uint32_t mbefore = mb + wm + rm[k];   // movers before slot i
+        const uint32_t dbefore = db + wd + rd[k];   // removed atoms before slot i
+        if (kind[k] == 1) {
             mov_k[mbefore] = kn[k];
             mov_v[mbefore] = (uint32_t)i;
+            mov_p[mbefore] = (uint32_t)i - mbefore - dbefore;   // stays with a smaller slot
+        } else if (kind[k] == 2) {
+            const uint32_t o = (uint32_t)n - dall + dbefore;
+            out_k[o] = kn[k];
+            out_v[o] = (uint32_t)i;
         } else {
-            const uint32_t sp = (uint32_t)i - mbefore;
+            const uint32_t sp = (uint32_t)i - mbefore - dbefore;
             stay_k[sp] = kn[k];
             stay_v[sp] = (uint32_t)i;
         }
     }
     if ((int)blockIdx.x == last_tile && tid == 0) {
-        const int movers = (int)(before + block_movers);
-        counts[0] = n - movers;
+        const int movers = (int)(mb + bm_all);
+        counts[0] = n - movers - (int)dall;
         counts[1] = movers;
+        counts[2] = (int)dall;
     }
 }
 
-// (key, slot) order; slots of the two runs are distinct.
+// (key, slot) order; slots are distinct.
 __device__ __forceinline__ bool kv_less(uint32_t ka, uint32_t va, uint32_t kb, uint32_t vb)
 {
     return ka < kb || (ka == kb && va < vb);
 }
 
-// Merge-path split of every output tile boundary: split[t] = stay elements
-// among the first min(t * kMergeTile, total) outputs.
-__global__ __launch_bounds__(256) void k_merge_partition(
+// Insertion point q of every mover among the stays (the number of stays that
+// precede it), and its slot in q's bucket.  The search gallops from the
+// number of stays with a smaller slot (where the mover sat before its key
+// changed): a move to the next cell along x lands within a few places,
+// along y or z within about Nc or Nc^2 cells' worth of atoms.  mov_r holds
+// that start on entry and the bucket slot on exit.
+__global__ __launch_bounds__(256) void k_ins_locate(
     const uint32_t *__restrict__ stay_k, const uint32_t *__restrict__ stay_v,
     const uint32_t *__restrict__ mov_k, const uint32_t *__restrict__ mov_v,
-    const int *__restrict__ counts, int *__restrict__ split, int n_bounds)
+    const int *__restrict__ counts, uint32_t *__restrict__ mov_q, uint32_t *__restrict__ mov_r,
+    uint32_t *__restrict__ bcnt)
 {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= n_bounds) return;
+    const int j = blockIdx.x * 256 + threadIdx.x;
     const int s = counts[0], m = counts[1];
-    const long long dd = (long long)t * kMergeTile;
-    const int d = (int)(dd < (long long)(s + m) ? dd : (long long)(s + m));
-    int lo = d - m > 0 ? d - m : 0, hi = d < s ? d : s;
+    if (blockIdx.x * 256 >= m) return;   // whole block leaves together
+    const bool valid = j < m;
+    const uint32_t k = valid ? mov_k[j] : 0u, v = valid ? mov_v[j] : 0u;
+    const int p0 = valid ? (int)mov_r[j] : 0;
+    int lo, hi;   // the insertion point lies in [lo, hi]
+    if (!valid) {
+        lo = hi = 0;
+    } else if (p0 < s && kv_less(stay_k[p0], stay_v[p0], k, v)) {
+        int off = 1;
+        while (p0 + off < s && kv_less(stay_k[p0 + off], stay_v[p0 + off], k, v)) off <<= 1;
+        lo = p0 + (off >> 1) + 1;
+        hi = min(p0 + off, s);
+    } else {
+        int off = 1;
+        while (p0 - off >= 0 && !kv_less(stay_k[p0 - off], stay_v[p0 - off], k, v)) off <<= 1;
+        lo = p0 - off < 0 ? 0 : p0 - off + 1;
+        hi = off == 1 ? p0 : p0 - (off >> 1);
+    }
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        const int j = d - 1 - mid;
-        if (kv_less(stay_k[mid], stay_v[mid], mov_k[j], mov_v[j])) lo = mid + 1; else hi = mid;
+        if (kv_less(stay_k[mid], stay_v[mid], k, v)) lo = mid + 1; else hi = mid;
     }
-    split[t] = lo;
+    // bucket slot: lanes of a wave with the same bucket share one atomic
+    // (nearby movers mostly share buckets)
+    const uint32_t bkt = (uint32_t)lo >> kStayShift;
+    const uint64_t peers = match_bits<kBucketBits>(bkt, __ballot(valid));
+    const int leader = __ffsll((unsigned long long)peers) - 1;
+    uint32_t base = 0;
+    if (valid && (int)(threadIdx.x & (kWave - 1)) == leader)
+        base = atomicAdd(&bcnt[bkt], (uint32_t)__popcll(peers));
+    base = __shfl(base, leader < 0 ? 0 : leader, kWave);
+    if (valid) {
+        mov_q[j] = (uint32_t)lo;
+        mov_r[j] = base + (uint32_t)__popcll(peers & lanemask_lt());
+    }
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_merge(
-    const uint32_t *__restrict__ stay_k, const uint32_t *__restrict__ stay_v,
-    const uint32_t *__restrict__ mov_k, const uint32_t *__restrict__ mov_v,
-    const int *__restrict__ counts, const int *__restrict__ split, uint32_t *__restrict__ out_k,
-    uint32_t *__restrict__ out_v)
+// Exclusive scan of the bucket counts into LDS (every block computes it).
+__device__ __forceinline__ void bucket_scan(const uint32_t *__restrict__ bcnt, int nbk, uint32_t *ex,
+                                            uint32_t *tmp)
 {
-    __shared__ uint32_t sk[kMergeTile];
-    __shared__ uint32_t sv[kMergeTile];
-    const int s = counts[0], m = counts[1], total = s + m;
-    const int d0 = blockIdx.x * kMergeTile;
-    if (d0 >= total) return;
-    const int d1 = min(d0 + kMergeTile, total);
-    const int i0 = split[blockIdx.x], i1 = split[blockIdx.x + 1];
-    const int j0 = d0 - i0, j1 = d1 - i1;
-    const int na = i1 - i0, nb = j1 - j0;
-    for (int q = threadIdx.x; q < na; q += kSortBlock) { sk[q] = stay_k[i0 + q]; sv[q] = stay_v[i0 + q]; }
-    for (int q = threadIdx.x; q < nb; q += kSortBlock) { sk[na + q] = mov_k[j0 + q]; sv[na + q] = mov_v[j0 + q]; }
+    const int per = (nbk + 255) / 256;
+    const int b0 = threadIdx.x * per;
+    uint32_t s = 0;
+    for (int i = 0; i < per; ++i) if (b0 + i < nbk) s += bcnt[b0 + i];
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan<256>(s, tmp, &tot);
+    for (int i = 0; i < per; ++i) if (b0 + i < nbk) { ex[b0 + i] = run; run += bcnt[b0 + i]; }
     __syncthreads();
-    // this thread's outputs: [diag, diag + kMergeItems) of the tile
-    const int diag = min((int)threadIdx.x * kMergeItems, na + nb);
-    int lo = diag - nb > 0 ? diag - nb : 0, hi = diag < na ? diag : na;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        const int j = na + diag - 1 - mid;
-        if (kv_less(sk[mid], sv[mid], sk[j], sv[j])) lo = mid + 1; else hi = mid;
+}
+
+constexpr int kPlaceBlocks = 256;
+
+__global__ __launch_bounds__(256) void k_ins_place(
+    const uint32_t *__restrict__ mov_k, const uint32_t *__restrict__ mov_v,
+    const uint32_t *__restrict__ mov_q, const uint32_t *__restrict__ mov_r,
+    const int *__restrict__ counts, const uint32_t *__restrict__ bcnt,
+    uint32_t *__restrict__ bk_k, uint32_t *__restrict__ bk_v, uint32_t *__restrict__ bk_q,
+    uint32_t *__restrict__ bex)
+{
+    __shared__ uint32_t ex[kMaxBuckets];
+    __shared__ uint32_t tmp[256 / kWave + 1];
+    const int s = counts[0], m = counts[1];
+    const int nbk = (s >> kStayShift) + 1;
+    bucket_scan(bcnt, nbk, ex, tmp);
+    if (blockIdx.x == 0)   // bucket offsets for the merge
+        for (int i = threadIdx.x; i < nbk; i += 256) bex[i] = ex[i];
+    if (m == 0) return;
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) {
+        const uint32_t q = mov_q[j];
+        const uint32_t o = ex[q >> kStayShift] + mov_r[j];
+        bk_k[o] = mov_k[j];
+        bk_v[o] = mov_v[j];
+        bk_q[o] = q;
     }
-    int a = lo, b = na + diag - lo;
-    uint32_t rk[kMergeItems], rv[kMergeItems];
-#pragma unroll
-    for (int q = 0; q < kMergeItems; ++q) {
-        const bool take_a = a < na && (b >= na + nb || kv_less(sk[a], sv[a], sk[b], sv[b]));
-        const int src = take_a ? a : b;
-        rk[q] = src < na + nb ? sk[src] : 0u;
-        rv[q] = src < na + nb ? sv[src] : 0u;
-        a += take_a;
-        b += !take_a;
+}
+
+// Sort of one bucket that does not fit the LDS table, by one block, in
+// global scratch: chunks of kBucketLds sorted in LDS (bitonic network), then
+// pairwise merges of runs (merge path per thread).  Used only when a great
+// many atoms land between the same two stays (e.g. nearly all atoms change
+// cell in one step); O(b log b) work for one block.  Leaves the sorted bucket
+// in (k0, v0, q0) or (k1, v1, q1); returns the one that holds it.
+__device__ int big_bucket_sort(uint32_t *k0, uint32_t *v0, uint32_t *q0, uint32_t *k1, uint32_t *v1,
+                               uint32_t *q1, int b, uint32_t *lk, uint32_t *lv, uint32_t *lq)
+{
+    const int tid = threadIdx.x;
+    for (int c0 = 0; c0 < b; c0 += kBucketLds) {
+        const int c = min(kBucketLds, b - c0);
+        for (int e = tid; e < kBucketLds; e += 256) {
+            const bool in = e < c;
+            lk[e] = in ? k0[c0 + e] : 0xFFFFFFFFu;
+            lv[e] = in ? v0[c0 + e] : 0xFFFFFFFFu;
+            lq[e] = in ? q0[c0 + e] : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        for (int size = 2; size <= kBucketLds; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int pr = tid; pr < kBucketLds / 2; pr += 256) {
+                    const int i = 2 * pr - (pr & (stride - 1));   // lower index of the pair
+                    const int j = i + stride;
+                    const bool up = (i & size) == 0;
+                    const bool gt = kv_less(lk[j], lv[j], lk[i], lv[i]);
+                    if (gt == up) {
+                        uint32_t t = lk[i]; lk[i] = lk[j]; lk[j] = t;
+                        t = lv[i]; lv[i] = lv[j]; lv[j] = t;
+                        t = lq[i]; lq[i] = lq[j]; lq[j] = t;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int e = tid; e < c; e += 256) { k0[c0 + e] = lk[e]; v0[c0 + e] = lv[e]; q0[c0 + e] = lq[e]; }
+        __syncthreads();
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kMergeItems; ++q) {
-        const int o = (int)threadIdx.x * kMergeItems + q;
-        if (o < na + nb) { sk[o] = rk[q]; sv[o] = rv[q]; }
+    int src = 0;
+    for (int wdt = kBucketLds; wdt < b; wdt <<= 1) {
+        const uint32_t *sk = src ? k1 : k0, *sv = src ? v1 : v0, *sq = src ? q1 : q0;
+        uint32_t *dk = src ? k0 : k1, *dv = src ? v0 : v1, *dq = src ? q0 : q1;
+        for (int lo = 0; lo < b; lo += 2 * wdt) {
+            const int na = min(wdt, b - lo), nb = max(0, min(wdt, b - lo - wdt));
+            const int len = na + nb;
+            const int per = (len + 255) / 256;
+            const int d0 = min(tid * per, len), d1 = min(d0 + per, len);
+            // merge path split of output d0 between run a = [lo, lo+na), b = [lo+na, ...)
+            int l = max(0, d0 - nb), h = min(d0, na);
+            while (l < h) {
+                const int mid = (l + h) >> 1;
+                const int jb = lo + na + d0 - 1 - mid;
+                if (kv_less(sk[lo + mid], sv[lo + mid], sk[jb], sv[jb])) l = mid + 1; else h = mid;
+            }
+            int ia = l, ib = d0 - l;
+            for (int d = d0; d < d1; ++d) {
+                const bool ta = ia < na && (ib >= nb || kv_less(sk[lo + ia], sv[lo + ia],
+                                                                 sk[lo + na + ib], sv[lo + na + ib]));
+                const int sidx = ta ? lo + ia : lo + na + ib;
+                dk[lo + d] = sk[sidx]; dv[lo + d] = sv[sidx]; dq[lo + d] = sq[sidx];
+                ia += ta; ib += !ta;
+            }
+        }
+        __syncthreads();   // this pass's writes are visible to the whole block
+        src ^= 1;
     }
-    __syncthreads();
-    for (int q = threadIdx.x; q < na + nb; q += kSortBlock) {
-        out_k[d0 + q] = sk[q];
-        out_v[d0 + q] = sv[q];
+    return src;
+}
+
+// One block per stay tile t: rank the tile's bucket, then write stays and
+// movers to out[] at their final positions (a contiguous window).  Buckets
+// larger than the LDS table are sorted in global scratch by the block.
+__global__ __launch_bounds__(256) void k_ins_merge(
+    const uint32_t *__restrict__ stay_k, const uint32_t *__restrict__ stay_v,
+    const uint32_t *__restrict__ bk_k, const uint32_t *__restrict__ bk_v,
+    const uint32_t *__restrict__ bk_q, const int *__restrict__ counts,
+    const uint32_t *__restrict__ bcnt, const uint32_t *__restrict__ bex,
+    uint32_t *__restrict__ srt_k, uint32_t *__restrict__ srt_v,
+    uint32_t *__restrict__ srt_q, uint32_t *__restrict__ alt_k, uint32_t *__restrict__ alt_v,
+    uint32_t *__restrict__ alt_q, uint32_t *__restrict__ out_k, uint32_t *__restrict__ out_v)
+{
+    __shared__ uint32_t lk[kBucketLds], lv[kBucketLds], lq[kBucketLds];
+    __shared__ uint32_t sk[kBucketLds], sv[kBucketLds], sq[kBucketLds];
+    const int s = counts[0];
+    const int t = blockIdx.x;
+    if (t > (s >> kStayShift)) return;   // whole block leaves together
+    const int tid = threadIdx.x;
+    const uint32_t before = bex[t];   // movers in earlier buckets
+    const int b = (int)bcnt[t];
+    const int g0 = t * kStayTile;
+    const int ns = min(kStayTile, s - g0);
+    const uint32_t w0 = (uint32_t)g0 + before;   // first output slot of this tile
+    const uint32_t *bk = bk_k + before, *bv = bk_v + before, *bq = bk_q + before;
+    const uint32_t *rq;   // bucket q values in rank order
+    if (b <= kBucketLds) {
+        for (int e = tid; e < b; e += 256) { lk[e] = bk[e]; lv[e] = bv[e]; lq[e] = bq[e]; }
+        __syncthreads();
+        for (int e = tid; e < b; e += 256) {
+            const uint32_t ke = lk[e], ve = lv[e];
+            int r = 0;
+            for (int f = 0; f < b; ++f) r += kv_less(lk[f], lv[f], ke, ve);
+            sk[r] = ke; sv[r] = ve; sq[r] = lq[e];
+        }
+        __syncthreads();
+        rq = sq;
+        for (int r = tid; r < b; r += 256) {
+            const uint32_t o = w0 + (sq[r] - (uint32_t)g0) + (uint32_t)r;
+            out_k[o] = sk[r];
+            out_v[o] = sv[r];
+        }
+    } else {
+        // copy the bucket to scratch and sort it there
+        uint32_t *ak = srt_k + before, *av = srt_v + before, *aq = srt_q + before;
+        uint32_t *ck = alt_k + before, *cv = alt_v + before, *cq = alt_q + before;
+        for (int e = tid; e < b; e += 256) { ak[e] = bk[e]; av[e] = bv[e]; aq[e] = bq[e]; }
+        __syncthreads();
+        const int which = big_bucket_sort(ak, av, aq, ck, cv, cq, b, lk, lv, lq);
+        const uint32_t *gk = which ? ck : ak, *gv = which ? cv : av, *gq = which ? cq : aq;
+        rq = gq;
+        for (int r = tid; r < b; r += 256) {
+            const uint32_t o = w0 + (gq[r] - (uint32_t)g0) + (uint32_t)r;
+            out_k[o] = gk[r];
+            out_v[o] = gv[r];
+        }
+    }
+    // stay g0 + li moves up by the bucket movers inserted at or before it
+    for (int li = tid; li < ns; li += 256) {
+        const uint32_t g = (uint32_t)(g0 + li);
+        int lo = 0, hi = b;   // first rank whose q exceeds g
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (rq[mid] <= g) lo = mid + 1; else hi = mid;
+        }
+        const uint32_t o = w0 + (uint32_t)li + (uint32_t)lo;
+        out_k[o] = stay_k[g];
+        out_v[o] = stay_v[g];
     }
 }
 
 size_t resort_scratch_bytes(int n_cap)
 {
     const size_t n = (size_t)(n_cap > 0 ? n_cap : 1);
-    const size_t tiles = (size_t)div_up((long long)n, kSplitTile) + 1;
-    return 8 * n * sizeof(uint32_t) + (tiles + 1) * sizeof(uint32_t) + (tiles + 2) * sizeof(int) +
-           4 * sizeof(int) + 12 * 256;
+    const size_t tiles = (size_t)div_up((long long)n, kSplitTile) + 2;
+    return 15 * n * sizeof(uint32_t) + 5 * tiles * sizeof(uint32_t) + 8 * sizeof(int) + 24 * 256;
 }
 
 void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem)
@@ -389,32 +609,42 @@ void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem)
     r.n_cap = (int)n;
     r.tiles = div_up((long long)n, kSplitTile);
     char *p = align((char *)mem);
-    uint32_t **arrays[8] = {&r.stay_k, &r.stay_v, &r.mov_k, &r.mov_v,
-                            &r.movs_k, &r.movs_v, &r.alt_k, &r.alt_v};
+    uint32_t **arrays[15] = {&r.stay_k, &r.stay_v, &r.mov_k, &r.mov_v, &r.mov_q, &r.mov_r,
+                             &r.bk_k,   &r.bk_v,   &r.bk_q,  &r.srt_k, &r.srt_v, &r.srt_q,
+                             &r.alt_k,  &r.alt_v,  &r.spare};
     for (auto a : arrays) { *a = (uint32_t *)p; p = align(p + n * sizeof(uint32_t)); }
-    r.tile_cnt = (uint32_t *)p; p = align(p + (size_t)(r.tiles + 1) * sizeof(uint32_t));
-    r.split = (int *)p; p = align(p + (size_t)(r.tiles + 2) * sizeof(int));
+    r.tile_cnt = (uint32_t *)p; p = align(p + (size_t)(r.tiles + 2) * sizeof(uint32_t));
+    r.tile_dead = (uint32_t *)p; p = align(p + (size_t)(r.tiles + 2) * sizeof(uint32_t));
+    r.bcnt = (uint32_t *)p; p = align(p + (size_t)(r.tiles + 2) * sizeof(uint32_t));
+    r.bex = (uint32_t *)p; p = align(p + (size_t)(r.tiles + 2) * sizeof(uint32_t));
     r.counts = (int *)p;
 }
 
+bool resort_supported(int n_cap)
+{
+    // the place kernel keeps one LDS entry per stay tile
+    return div_up(n_cap > 0 ? n_cap : 1, kStayTile) + 1 <= kMaxBuckets;
+}
+
 void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const int *n_dev,
-                        int key_bits, const ResortScratch &r, const SortScratch &s,
-                        uint32_t *out_k, uint32_t *out_v, hipStream_t stream, Profiler *prof)
+                        uint32_t dead_key, const uint32_t *blk_moved, const uint32_t *blk_dead,
+                        int move_block, const ResortScratch &r, uint32_t *out_k, uint32_t *out_v,
+                        hipStream_t stream, Profiler *prof)
 {
     const double n = (double)r.n_cap;
-    DSMC_LAUNCH(prof, "resort_count", n * 8.0, k_split_count, r.tiles, kSortBlock, stream,
-                key_new, key_old, n_dev, r.tile_cnt);
-    DSMC_LAUNCH(prof, "resort_split", n * 16.0, k_split_scatter, r.tiles, kSortBlock, stream,
-                key_new, key_old, n_dev, r.tile_cnt, r.stay_k, r.stay_v, r.mov_k, r.mov_v,
-                r.counts);
-    radix_sort(r.mov_k, r.mov_v, r.alt_k, r.alt_v, r.movs_k, r.movs_v, r.counts + 1, key_bits, s,
-               stream, prof);
-    const int bounds = r.tiles + 1;  // merge tiles == split tiles (total <= n_cap)
-    DSMC_LAUNCH(prof, "resort_partition", (double)bounds * 64.0, k_merge_partition,
-                div_up(bounds, 256), 256, stream, r.stay_k, r.stay_v, r.movs_k, r.movs_v,
-                r.counts, r.split, bounds);
-    DSMC_LAUNCH(prof, "resort_merge", n * 16.0, k_merge, r.tiles, kSortBlock, stream, r.stay_k,
-                r.stay_v, r.movs_k, r.movs_v, r.counts, r.split, out_k, out_v);
+    DSMC_LAUNCH(prof, "resort_tiles", 0.0, k_ins_tiles, div_up(r.tiles, kTilesBlock), kTilesBlock,
+                stream, blk_moved, blk_dead, kSplitTile / move_block, n_dev, r.tile_cnt,
+                r.tile_dead, r.counts);
+    DSMC_LAUNCH(prof, "resort_split", n * 16.0, k_ins_split, r.tiles, kSortBlock, stream,
+                key_new, key_old, n_dev, dead_key, r.tiles, r.tile_cnt, r.tile_dead, r.stay_k,
+                r.stay_v, r.mov_k, r.mov_v, r.mov_r, out_k, out_v, r.counts, r.bcnt);
+    DSMC_LAUNCH(prof, "resort_locate", 0.0, k_ins_locate, div_up(r.n_cap, 256), 256, stream,
+                r.stay_k, r.stay_v, r.mov_k, r.mov_v, r.counts, r.mov_q, r.mov_r, r.bcnt);
+    DSMC_LAUNCH(prof, "resort_place", 0.0, k_ins_place, kPlaceBlocks, 256, stream, r.mov_k,
+                r.mov_v, r.mov_q, r.mov_r, r.counts, r.bcnt, r.bk_k, r.bk_v, r.bk_q, r.bex);
+    DSMC_LAUNCH(prof, "resort_merge", n * 16.0, k_ins_merge, div_up(r.n_cap, kStayTile) + 1, 256,
+                stream, r.stay_k, r.stay_v, r.bk_k, r.bk_v, r.bk_q, r.counts, r.bcnt, r.bex, r.srt_k,
+                r.srt_v, r.srt_q, r.alt_k, r.alt_v, r.spare, out_k, out_v);
 }
 
 // ---------------------------------------------------------------------------

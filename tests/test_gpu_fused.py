@@ -145,20 +145,53 @@ def test_incremental_sort_is_the_stable_sort():
 
 
 def test_incremental_sort_equals_full_sort():
-    """Whole batched runs (evaporation on, graph replay) with the incremental
-    and the full sort are bitwise identical."""
+    """Whole batched runs (evaporation on, graph replay, a block-size autotune
+    half way) with the incremental and the full sort are bitwise identical."""
     run = _batch_run()
     out = {}
     for mode in ("incremental", "full"):
         print(f"batched run, sort mode {mode}", flush=True)
-        with Simulation(run.with_(sort_mode=mode)) as sim:
+        with Simulation(run.with_(sort_mode=mode, block_size=64)) as sim:
             sim.init_distribution()
-            sim.step(41)
+            sim.step(20)
+            sim.autotune()   # may change the move block size between steps
+            sim.step(21)
             out[mode] = [sim.copy_out(f) for f in (_abi.FIELD_POS, _abi.FIELD_VEL,
                                                    _abi.FIELD_ATOM_ID)]
             out[mode].append(sim.obs_log()["collisions"].copy())
             out[mode].append(sim.obs_log()["n_evap"].copy())
     assert int(out["full"][4].sum()) > 0
+    for a, b in zip(out["incremental"], out["full"]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_incremental_sort_with_crowded_gap():
+    """Atoms aimed at one cell: on the second step nearly every atom changes
+    cell into the same cell, so one gap between two stays receives ~all the
+    movers (the block-level merge sort path of the re-sort), and on the third
+    step they leave it again.  Bitwise equal to the full radix sort."""
+    n, L, nc, dt = 100000, 1e-3, 16, 1e-6
+    trap = configs.Trap(kind="box", n_atoms=n, nc=(nc, nc, nc), half_width=(L / 2,) * 3,
+                        T=20e-6, mu=0.0, n_real=1e10)
+    run = configs.Run(traps=[trap], precision="fp32", dt=dt, seed=9, sigma=0.0,
+                      name="crowded-gap")
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(-0.4 * L, 0.4 * L, (n, 3))
+    target = np.array([0.5, 0.5, 0.5]) * (L / nc)   # centre of a cell
+    vel = (target - pos) / (2 * dt)
+    out = {}
+    for mode in ("incremental", "full"):
+        print(f"crowded gap, sort mode {mode}", flush=True)
+        with Simulation(run.with_(sort_mode=mode)) as sim:
+            sim.set_state(pos, vel)
+            sim.step(2)
+            keys = sim.copy_out(_abi.FIELD_CELL_ID)
+            crowd = int(np.max(np.bincount(keys.astype(np.int64))))
+            sim.step(2)
+            out[mode] = [sim.copy_out(f) for f in (_abi.FIELD_POS, _abi.FIELD_VEL,
+                                                   _abi.FIELD_ATOM_ID)]
+        print(f"largest cell after two steps: {crowd}", flush=True)
+        assert crowd > n // 2
     for a, b in zip(out["incremental"], out["full"]):
         np.testing.assert_array_equal(a, b)
 
