@@ -307,6 +307,37 @@ __device__ __forceinline__ float rel_speed(const FusedPtrs &s, uint32_t a, uint3
     return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
+// Isotropic unit vector for the FP32 form: Marsaglia's (1972) disc method in
+// FP32 with two attempts per Philox block (acceptance pi/4 each), then
+// normalised in FP64 so that |n| = 1 to FP64 rounding and the collision
+// conserves energy to FP64 rounding.  Falls back to +z after 32 rejections
+// (probability 0.215^32).
+__device__ __forceinline__ void sphere_point_f(unsigned long long seed, uint32_t stream,
+                                               uint32_t cell, uint32_t step, uint32_t tr,
+                                               double &nx, double &ny, double &nz)
+{
+    float fx = 0.0f, fy = 0.0f, fz = 1.0f;
+#pragma unroll 1
+    for (uint32_t k = 0; k < 16; ++k) {
+        const U32x4 o = draw(seed, stream, cell, step, tr, kPurposeCollide, 2 + k);
+        float u1 = 2.0f * u01f(o.x) - 1.0f, u2 = 2.0f * u01f(o.y) - 1.0f;
+        float q = u1 * u1 + u2 * u2;
+        if (!(q < 1.0f)) {
+            u1 = 2.0f * u01f(o.z) - 1.0f;
+            u2 = 2.0f * u01f(o.w) - 1.0f;
+            q = u1 * u1 + u2 * u2;
+        }
+        if (q < 1.0f) {
+            const float r = 2.0f * sqrtf(1.0f - q);
+            fx = u1 * r; fy = u2 * r; fz = 1.0f - 2.0f * q;
+            break;
+        }
+    }
+    nx = fx; ny = fy; nz = fz;
+    const double inv = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
+    nx *= inv; ny *= inv; nz *= inv;
+}
+
 // Accepted collision of atoms a and b of trial tr, in FP64 from value +
 // remainder, stored back as float + remainder (collisions.cu:426-436).
 __device__ __forceinline__ void scatter_pair(const CollideParams &cp, uint32_t stream, uint32_t cell,
@@ -323,7 +354,7 @@ __device__ __forceinline__ void scatter_pair(const CollideParams &cp, uint32_t s
     const double magVrel = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
     const double cx = 0.5 * (vax + vbx), cy = 0.5 * (vay + vby), cz = 0.5 * (vaz + vbz);
     double nx, ny, nz;
-    sphere_point(cp.seed, stream, cell, step, (uint32_t)tr, kPurposeCollide, 2, nx, ny, nz);
+    sphere_point_f(cp.seed, stream, cell, step, (uint32_t)tr, nx, ny, nz);
     const double wx = magVrel * nx, wy = magVrel * ny, wz = magVrel * nz;
     split_df(cx - 0.5 * wx, s.v[0][a], s.cv[0][a]);
     split_df(cy - 0.5 * wy, s.v[1][a], s.cv[1][a]);
@@ -407,8 +438,10 @@ constexpr int kCollItems = 4;  // slots per lane scanned for cell heads
 constexpr int kWaveTrialsDefault = 16;
 
 // Cells with more than `wave_trials` trials are run by the whole wave after
-// the lanes have run the lighter cells one per lane.
-__global__ __launch_bounds__(kFBlock) void k_collide_fused(
+// the lanes have run the lighter cells one per lane.  Five waves per SIMD:
+// the kernel is bound by the latency of its dependent loads (key, sigma
+// v_r,max, permutation, velocities), and 96 registers cost a few spills only.
+__global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     const TrapDev *__restrict__ traps, const int *__restrict__ key_base, int T, CollideParams cp,
     const uint32_t *__restrict__ key, const uint32_t *__restrict__ perm,
     const int *__restrict__ n_dev, FusedPtrs s, double *__restrict__ sigvrmax,
