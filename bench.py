@@ -38,14 +38,20 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per atom (or per pass) of each kernel of the FP32 form.
-#   move_fused: read perm (4) + 12 floats of state (48) + label (4);
-#               write 12 floats (48) + label (4) + next cell key (4)
+#   move_fused: read perm (4) + 12 floats of state (48) + label (4) + the
+#               previous sort key of the slot (4); write 12 floats (48) +
+#               label (4) + next cell key (4)
+#   resort_split: read new and previous key (8), write the unchanged atoms'
+#               (key, slot) (8); resort_merge: read those (8), write the sorted
+#               (key, slot) (8) -- movers (a few percent) not counted
 #   radix_downsweep: read key + value, write key + value (first pass reads no
 #               value: identity); radix_upsweep: read key
 #   table_fused: read key; collide_fused: read key (per slot) -- per-cell work
 #               is counted by the measured traffic, not here.
 PER_ATOM_BYTES = {
-    "move_fused": 112.0,
+    "move_fused": 116.0,
+    "resort_split": 16.0,
+    "resort_merge": 16.0,
     "radix_downsweep": 16.0,
     "radix_upsweep": 4.0,
     "table_fused": 4.0,
@@ -276,18 +282,19 @@ def main():
             f"max |dE/E| {b['energy_drift']:.2e}")
         result["batched"] = b
         kernels, n_mean = profile_leg(sim, args.profile_steps)
-        info = sim.info()
         sim.close()
         dom = max((k for k in kernels if "alg_bytes" in kernels[k]),
                   key=lambda k: kernels[k]["total_ms"])
         d = kernels[dom]
         achieved = d["alg_bytes"] / (d["avg_us"] * 1e-6) / 1e9
         tr = traffic if (traffic and traffic["kernel"] == dom) else None
+        cal = copy_calibration()
         roof = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=round(tr["bytes"]) if tr else None,
                     alg_bytes_per_launch=round(d["alg_bytes"]), avg_launch_us=d["avg_us"],
-                    radix_passes=info.radix_passes)
+                    copy_calibration_GBs=round(cal["GBs"], 1),
+                    frac_of_copy=round(achieved / cal["GBs"], 4))
         if tr:
             roof["traffic_detail"] = {k: (round(v, 1) if isinstance(v, float) else v)
                                       for k, v in tr.items()}
