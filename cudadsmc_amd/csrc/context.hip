@@ -120,6 +120,8 @@ struct dsmc_ctx {
     int obs_cap = 0;
     int block = 256;
     bool have_sort = false;         // FP32: perm valid for the next move
+    bool tbl_valid = false;         // FP32: trap_start and the alive count describe the
+                                    // next sort's output (set by a sorted move's finalize)
     bool srt_valid = false;         // FP32: srt_k[srt_cur][i] is the key of slot i of the
                                     // current state (slots in sorted order of those keys)
     std::vector<void *> allocs;
@@ -555,6 +557,7 @@ static void fused_index(dsmc_ctx *x)
 {
     launch_index_fused(x->d_traps, x->trap_start, x->T, x->fs[x->cur], x->key_a, x->n_dev + x->cur,
                        x->n_cap, x->stream, &x->prof);
+    x->tbl_valid = false;   // the new keys decide the trap ranges of the next sort
 }
 
 // Stable sort of the keys in key_a into the other sort buffer.  When the
@@ -585,10 +588,19 @@ static void fused_table(dsmc_ctx *x)
     tp.dead_key = x->dead_key;
     launch_table_fused(x->sorted_key, x->d_key_base, tp, x->n_dev + x->cur, x->n_dev + (x->cur ^ 1),
                        x->trap_start, x->n_cap, x->stream, &x->prof);
+    x->tbl_valid = true;
+}
+
+// The trap ranges and alive count of the sorted order: kept by the previous
+// sorted move's finalize, else scanned from the sorted keys.
+static void fused_need_table(dsmc_ctx *x)
+{
+    if (x->have_sort && !x->tbl_valid) fused_table(x);
 }
 
 static void fused_collide(dsmc_ctx *x)
 {
+    fused_need_table(x);
     launch_collide_fused(x->d_traps, x->d_key_base, x->T, collide_params(x), x->sorted_key,
                          x->perm, x->n_dev + (x->cur ^ 1), x->fs[x->cur], x->sigvrmax,
                          x->coll_accum, x->step_dev, x->n_cap, x->stream, &x->prof);
@@ -596,6 +608,7 @@ static void fused_collide(dsmc_ctx *x)
 
 static void fused_move(dsmc_ctx *x)
 {
+    fused_need_table(x);
     // after a sort the alive count of the sorted order is n_dev[cur ^ 1]
     const int *n = x->have_sort ? x->n_dev + (x->cur ^ 1) : x->n_dev + x->cur;
     MoveParams mp = move_params(x);
@@ -607,8 +620,8 @@ static void fused_move(dsmc_ctx *x)
                       x->evap_pos_f, x->evap_vel_f, x->rec, x->step_dev, x->n_cap, x->stream,
                       &x->prof);
     launch_obs_finalize(x->T, x->max_trap_atoms, x->block, x->cfg.mass, x->trap_start, x->rec,
-                        x->coll_accum,
-                        x->done, x->step_dev, x->obs_log, x->obs_cap, x->stream, &x->prof);
+                        x->coll_accum, x->done, x->step_dev, x->obs_log, x->obs_cap,
+                        x->have_sort ? x->n_dev + x->cur : nullptr, x->stream, &x->prof);
     if (!x->have_sort)  // keep the count with the state it describes
         DSMC_HIP(hipMemcpyAsync(x->n_dev + (x->cur ^ 1), x->n_dev + x->cur, sizeof(int),
                                 hipMemcpyDeviceToDevice, x->stream));
@@ -616,6 +629,7 @@ static void fused_move(dsmc_ctx *x)
     // and has counted the changed keys; after an unsorted move the next sort
     // is a full one
     x->srt_valid = x->have_sort;
+    x->tbl_valid = x->have_sort;
     x->have_sort = false;
     x->cur ^= 1;
 }
@@ -623,7 +637,6 @@ static void fused_move(dsmc_ctx *x)
 static void fused_step(dsmc_ctx *x)
 {
     fused_sort(x);
-    fused_table(x);
     fused_collide(x);
     x->coll_steps++;
     fused_move(x);
@@ -776,6 +789,7 @@ static void reset_cells(dsmc_ctx *x)
     x->cur = 0;
     x->have_sort = false;
     x->srt_valid = false;
+    x->tbl_valid = true;    // all atoms alive, traps at their initial ranges
 }
 
 static InitParams init_params(const dsmc_ctx *x)

@@ -700,14 +700,16 @@ __global__ __launch_bounds__(BS) void k_move_fused(
 // adds the boundary term, appends the trap's observables to the log and
 // clears the per-step accumulators; the last trap advances the step.
 __global__ __launch_bounds__(256) void k_obs_finalize(
-    int T, int P, int move_block, double half_mass, const int *__restrict__ trap_start,
+    int T, int P, int move_block, double half_mass, int *__restrict__ trap_start,
     const int *__restrict__ rec_trap, const double *__restrict__ rec_sum, int rec_stride,
     double *__restrict__ lvl2, unsigned int *__restrict__ cnt, double *__restrict__ acc_over,
     unsigned long long *__restrict__ coll_accum, unsigned int *__restrict__ done,
-    long long *__restrict__ step_dev, dsmc_observables *__restrict__ obs_log, int obs_cap)
+    long long *__restrict__ step_dev, dsmc_observables *__restrict__ obs_log, int obs_cap,
+    int *__restrict__ n_next)
 {
     __shared__ double red[(256 / kWave) * kAccK];
-    __shared__ bool last;
+    __shared__ bool fin, last_trap;
+    __shared__ int tmp[256 / kWave + 1];
     const int p = blockIdx.x, t = blockIdx.y;
     const long long step = *step_dev;
     const int s0 = trap_start[t], s1 = trap_start[t + 1];
@@ -724,49 +726,83 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
         }
     }
     block_sum<256, kAccK>(a, red);
+    // one block per trap, or the last of the trap's P blocks, finalizes it
     double f[kAccK];
     if (P == 1) {
-        // one block per trap: its sum is the trap's total
-        if (threadIdx.x != 0) return;
 #pragma unroll
         for (int q = 0; q < kAccK; ++q) f[q] = a[q];
-    } else {
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0) fin = true;
+    } else if (threadIdx.x == 0) {
 #pragma unroll
-            for (int q = 0; q < kAccK; ++q)
-                __hip_atomic_store(&lvl2[((size_t)t * P + p) * kAccK + q], a[q], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
-            last = atomicAdd(&cnt[t], 1u) == (unsigned)(P - 1);
-        }
-        __syncthreads();
-        if (!last || threadIdx.x != 0) return;
+        for (int q = 0; q < kAccK; ++q)
+            __hip_atomic_store(&lvl2[((size_t)t * P + p) * kAccK + q], a[q], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
+        fin = atomicAdd(&cnt[t], 1u) == (unsigned)(P - 1);
+        if (fin) {
+            __threadfence();
 #pragma unroll
-        for (int q = 0; q < kAccK; ++q) f[q] = 0.0;
-        for (int pp = 0; pp < P; ++pp)
-            for (int q = 0; q < kAccK; ++q)
-                f[q] += __hip_atomic_load(&lvl2[((size_t)t * P + pp) * kAccK + q], __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-        cnt[t] = 0u;
+            for (int q = 0; q < kAccK; ++q) f[q] = 0.0;
+            for (int pp = 0; pp < P; ++pp)
+                for (int q = 0; q < kAccK; ++q)
+                    f[q] += __hip_atomic_load(&lvl2[((size_t)t * P + pp) * kAccK + q],
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            cnt[t] = 0u;
+        }
     }
-    for (int q = 0; q < kAccK; ++q) {
-        f[q] += acc_over[t * kAccK + q];
-        acc_over[t * kAccK + q] = 0.0;
+    __syncthreads();
+    if (!fin) return;   // whole block leaves together
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < kAccK; ++q) {
+            f[q] += acc_over[t * kAccK + q];
+            acc_over[t * kAccK + q] = 0.0;
+        }
+        const unsigned long long c = coll_accum[t];
+        coll_accum[t] = 0ull;
+        dsmc_observables &o = obs_log[(step % obs_cap) * T + t];
+        o.ek = half_mass * (f[kAccV2x] + f[kAccV2y] + f[kAccV2z]);
+        o.ep = f[kAccEp];
+        o.v2[0] = f[kAccV2x]; o.v2[1] = f[kAccV2y]; o.v2[2] = f[kAccV2z];
+        o.e_evap = f[kAccEevap];
+        o.n_alive = (long long)(s1 - s0);
+        o.n_evap = (long long)f[kAccNevap];
+        o.collisions = (long long)c;
+        o.step = step;
+        __threadfence();
+        last_trap = atomicAdd(done, 1u) == (unsigned)(T - 1);
     }
-    const unsigned long long c = coll_accum[t];
-    coll_accum[t] = 0ull;
-    dsmc_observables &o = obs_log[(step % obs_cap) * T + t];
-    o.ek = half_mass * (f[kAccV2x] + f[kAccV2y] + f[kAccV2z]);
-    o.ep = f[kAccEp];
-    o.v2[0] = f[kAccV2x]; o.v2[1] = f[kAccV2y]; o.v2[2] = f[kAccV2z];
-    o.e_evap = f[kAccEevap];
-    o.n_alive = (long long)(s1 - s0);
-    o.n_evap = (long long)f[kAccNevap];
-    o.collisions = (long long)c;
-    o.step = step;
+    __syncthreads();
+    if (!last_trap) return;
     __threadfence();
-    if (atomicAdd(done, 1u) == (unsigned)(T - 1)) {
+    // Last trap finalized: every log record of this step is written.  When
+    // the move followed a sort, the next sort puts the atoms removed now at
+    // the end, so trap t's range shrinks by its removed atoms: the next
+    // step's trap ranges and alive count follow without scanning the keys.
+    if (n_next) {
+        dsmc_observables *row = obs_log + (step % obs_cap) * T;
+        auto left = [&](int tt) {   // atoms of trap tt that stay alive
+            const long long na = __hip_atomic_load(&row[tt].n_alive, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            const long long ne = __hip_atomic_load(&row[tt].n_evap, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            return (int)(na - ne);
+        };
+        const int per = (T + 255) / 256;
+        const int t0 = min(T, (int)threadIdx.x * per), t1 = min(T, t0 + per);
+        int sum = 0;
+        for (int tt = t0; tt < t1; ++tt) sum += left(tt);
+        int total;
+        int run = block_exclusive_scan<256>(sum, tmp, &total);
+        for (int tt = t0; tt < t1; ++tt) {
+            trap_start[tt] = run;
+            run += left(tt);
+        }
+        if (threadIdx.x == 0) {
+            trap_start[T] = total;
+            *n_next = total;
+        }
+    }
+    if (threadIdx.x == 0) {
         *done = 0u;
         *step_dev = step + 1;
     }
@@ -910,9 +946,9 @@ void launch_move_fused(int block, const TrapDev *traps, const MoveParams &mp, co
 }
 
 void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, double mass,
-                         const int *trap_start,
+                         int *trap_start,
                          const ObsRecords &rec, unsigned long long *coll_accum, unsigned int *done,
-                         long long *step_dev, dsmc_observables *obs_log, int obs_cap,
+                         long long *step_dev, dsmc_observables *obs_log, int obs_cap, int *n_next,
                          hipStream_t st, Profiler *prof)
 {
     const int blocks = div_up(max_trap_atoms > 0 ? max_trap_atoms : 1, move_block) + 1;
@@ -920,7 +956,7 @@ void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, double
     DSMC_LAUNCH(prof, "obs_finalize", (double)n_traps * 64.0, k_obs_finalize, dim3(P, n_traps), 256,
                 st, n_traps, P, move_block, 0.5 * mass, trap_start, (const int *)rec.trap,
                 (const double *)rec.sum, rec.stride, rec.lvl2, rec.cnt, rec.acc_over, coll_accum,
-                done, step_dev, obs_log, obs_cap);
+                done, step_dev, obs_log, obs_cap, n_next);
 }
 
 void launch_gather_fused(const FusedPtrs &dst, const FusedPtrs &src, const uint32_t *map,

@@ -84,10 +84,13 @@ void launch_move_fused(int block, const TrapDev *traps, const MoveParams &mp, co
                        const int *n_dev, uint32_t *key_out, float *evap_pos, float *evap_vel,
                        const ObsRecords &rec, const long long *step_dev, int n_cap, hipStream_t st,
                        Profiler *prof);
+// When n_next is set (the move followed a sort), the last block also writes
+// the next step's trap ranges (trap_start) and alive count (*n_next): each
+// trap loses its atoms removed in this step, which the next sort puts last.
 void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, double mass,
-                         const int *trap_start,
+                         int *trap_start,
                          const ObsRecords &rec, unsigned long long *coll_accum, unsigned int *done,
-                         long long *step_dev, dsmc_observables *obs_log, int obs_cap,
+                         long long *step_dev, dsmc_observables *obs_log, int obs_cap, int *n_next,
                          hipStream_t st, Profiler *prof);
 void launch_gather_fused(const FusedPtrs &dst, const FusedPtrs &src, const uint32_t *map,
                          const int *n_dev, int n_cap, int spin, hipStream_t st, Profiler *prof);

@@ -196,6 +196,28 @@ def test_incremental_sort_with_crowded_gap():
         np.testing.assert_array_equal(a, b)
 
 
+def test_trap_ranges_after_evaporation():
+    """The trap ranges and alive count that the step keeps (from the atoms
+    each trap lost) describe the sorted order exactly: checked against the
+    sorted keys themselves and against the removal counts in the log."""
+    run = _batch_run()
+    key_base = np.cumsum([0] + [int(np.prod(t.nc)) + 1 for t in run.traps])
+    with Simulation(run) as sim:
+        sim.init_distribution()
+        sim.step(31)
+        sim.sort_arrays()
+        keys = sim.copy_out(_abi.FIELD_CELL_ID).astype(np.int64)
+        ts = sim.copy_out(_abi.FIELD_TRAP_START)
+        log = sim.obs_log()
+    removed = int(log["n_evap"].sum())
+    print(f"alive {len(keys)}, removed {removed}", flush=True)
+    assert removed > 0
+    assert len(keys) == run.n_atoms - removed
+    assert np.all(keys < key_base[-1])          # no removed atom among them
+    assert np.all(np.diff(keys) >= 0)
+    np.testing.assert_array_equal(ts, np.searchsorted(keys, key_base))
+
+
 def test_graph_replay_equals_eager():
     run = quad(40000, 30)
     with Simulation(run) as a, Simulation(run.with_(use_graph=False)) as b:
