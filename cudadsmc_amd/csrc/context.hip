@@ -118,7 +118,8 @@ struct dsmc_ctx {
     long long coll_steps = 0;
     int cur = 0;                    // state buffer holding the current state
     int obs_cap = 0;
-    int block = 256;
+    int block = 256;                // FP32 move: threads per block
+    int apt = 1;                    // FP32 move: atoms per thread
     bool have_sort = false;         // FP32: perm valid for the next move
     bool tbl_valid = false;         // FP32: trap_start and the alive count describe the
                                     // next sort's output (set by a sorted move's finalize)
@@ -547,7 +548,7 @@ static MoveParams move_params(const dsmc_ctx *x)
     mp.n_traps = x->T;
     mp.obs_cap = x->obs_cap;
     mp.dead_key = x->dead_key;
-    mp.key_old = nullptr;
+    mp.count_moved = 0;
     mp.blk_moved = x->blk_moved;
     mp.blk_dead = x->blk_dead;
     return mp;
@@ -570,7 +571,7 @@ static void fused_sort(dsmc_ctx *x)
     uint32_t *ok = x->srt_k[x->srt_cur ^ 1], *ov = x->srt_v[x->srt_cur ^ 1];
     if (x->srt_valid && x->cfg.sort_mode == DSMC_SORT_INCREMENTAL && resort_supported(x->n_cap))
         resort_incremental(x->key_a, x->srt_k[x->srt_cur], n, x->dead_key, x->blk_moved,
-                           x->blk_dead, x->block, x->res, ok, ov, x->stream, &x->prof);
+                           x->blk_dead, x->block * x->apt, x->res, ok, ov, x->stream, &x->prof);
     else
         radix_sort(x->key_a, nullptr, x->res.alt_k, x->res.alt_v, ok, ov, n, x->key_bits,
                    x->sort, x->stream, &x->prof);
@@ -614,12 +615,12 @@ static void fused_move(dsmc_ctx *x)
     MoveParams mp = move_params(x);
     // a sorted move also counts, per move block, the atoms whose key changed:
     // the input of the next step's incremental re-sort
-    if (x->have_sort) mp.key_old = x->sorted_key;
-    launch_move_fused(x->block, x->d_traps, mp, x->fs[x->cur], x->fs[x->cur ^ 1],
+    if (x->have_sort) mp.count_moved = 1;
+    launch_move_fused(x->block, x->apt, x->d_traps, mp, x->fs[x->cur], x->fs[x->cur ^ 1],
                       x->have_sort ? x->perm : nullptr, x->trap_start, n, x->key_a,
                       x->evap_pos_f, x->evap_vel_f, x->rec, x->step_dev, x->n_cap, x->stream,
                       &x->prof);
-    launch_obs_finalize(x->T, x->max_trap_atoms, x->block, x->cfg.mass, x->trap_start, x->rec,
+    launch_obs_finalize(x->T, x->max_trap_atoms, x->block * x->apt, x->cfg.mass, x->trap_start, x->rec,
                         x->coll_accum, x->done, x->step_dev, x->obs_log, x->obs_cap,
                         x->have_sort ? x->n_dev + x->cur : nullptr, x->stream, &x->prof);
     if (!x->have_sort)  // keep the count with the state it describes
@@ -1339,20 +1340,23 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
     MoveParams mp = move_params(ctx);
     mp.spin_loss = 0;
     mp.energy_cut = 0;
-    const int cands[5] = { 64, 128, 256, 512, 1024 };
+    // (threads per block, atoms per thread)
+    const int cands[9][2] = { { 64, 1 }, { 128, 1 }, { 256, 1 }, { 512, 1 }, { 1024, 1 },
+                              { 64, 2 }, { 128, 2 }, { 256, 2 }, { 512, 2 } };
     double best = 1e30;
-    int best_bs = ctx->block;
+    int best_bs = ctx->block, best_apt = ctx->apt;
     hipEvent_t b, e;
     DSMC_HIP(hipEventCreate(&b));
     DSMC_HIP(hipEventCreate(&e));
     const bool was = ctx->prof.enabled;
     ctx->prof.enabled = false;
-    for (int bs : cands) {
+    for (const auto &cand : cands) {
+        const int bs = cand[0], apt = cand[1];
         float f = 0.f;
         for (int rep = 0; rep < 2; ++rep) {  // first repetition warms up
             DSMC_HIP(hipEventRecord(b, ctx->stream));
             for (int k = 0; k < 5; ++k)
-                launch_move_fused(bs, ctx->d_traps, mp, ctx->fs[ctx->cur], ctx->fs[ctx->cur ^ 1],
+                launch_move_fused(bs, apt, ctx->d_traps, mp, ctx->fs[ctx->cur], ctx->fs[ctx->cur ^ 1],
                                   nullptr, ctx->trap_start, ctx->n_dev + ctx->cur, ctx->res.alt_k,
                                   ctx->evap_pos_f, ctx->evap_vel_f, ctx->rec, ctx->step_dev,
                                   ctx->n_cap, ctx->stream, nullptr);
@@ -1360,7 +1364,7 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
             DSMC_HIP(hipEventSynchronize(e));
             DSMC_HIP(hipEventElapsedTime(&f, b, e));
         }
-        if (f < best) { best = f; best_bs = bs; }
+        if (f < best) { best = f; best_bs = bs; best_apt = apt; }
     }
     ctx->prof.enabled = was;
     (void)hipEventDestroy(b);
@@ -1369,7 +1373,7 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
                             hipMemcpyDeviceToDevice, ctx->stream));
     DSMC_HIP(hipMemsetAsync(ctx->acc, 0, sizeof(double) * kAccK * ctx->T, ctx->stream));
     DSMC_HIP(hipStreamSynchronize(ctx->stream));
-    if (ctx->block != best_bs) {
+    if (ctx->block != best_bs || ctx->apt != best_apt) {
         if (ctx->graph_exec) {
             (void)hipGraphExecDestroy(ctx->graph_exec);
             (void)hipGraphDestroy(ctx->graph);
@@ -1380,6 +1384,7 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
         ctx->srt_valid = false;
     }
     ctx->block = best_bs;
+    ctx->apt = best_apt;
     if (block_size_out) *block_size_out = best_bs;
     DSMC_API_END(ctx)
 }

@@ -523,7 +523,135 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
 // observables accumulated per trap.  The last block to finish writes one log
 // record per trap and advances the device step counter.
 // ---------------------------------------------------------------------------
-template <int BS>
+// One atom of the fused move, in registers between its load and its update.
+struct MoveAtom {
+    float x, y, z, cx, cy, cz, vx, vy, vz, cvx, cvy, cvz, sx, sy, sz;
+    int id;
+    int t;          // trap
+    uint32_t kold;  // key before the move (when counting)
+};
+
+__device__ __forceinline__ void move_load(const MoveParams &mp, const FusedPtrs &A,
+                                          const TrapDev &tr, uint32_t j, MoveAtom &m)
+{
+    m.x = A.p[0][j]; m.y = A.p[1][j]; m.z = A.p[2][j];
+    m.cx = A.cp[0][j]; m.cy = A.cp[1][j]; m.cz = A.cp[2][j];
+    m.vx = A.v[0][j]; m.vy = A.v[1][j]; m.vz = A.v[2][j];
+    m.cvx = A.cv[0][j]; m.cvy = A.cv[1][j]; m.cvz = A.cv[2][j];
+    m.id = A.id[j];
+    m.sx = m.sy = m.sz = 0.0f;
+    if (mp.spin) { m.sx = A.s[0][j]; m.sy = A.s[1][j]; m.sz = A.s[2][j]; }
+    // key of the slot before the move: the stored position gives exactly the
+    // key the last sort used (recomputed instead of read)
+    m.kold = 0;
+    if (mp.count_moved)
+        m.kold = (uint32_t)(tr.key_base + cell_of_fast(tr, (double)m.x + (double)m.cx,
+                                                       (double)m.y + (double)m.cy,
+                                                       (double)m.z + (double)m.cz));
+}
+
+// Spin substeps, Kahan velocity Verlet, wrap, evaporation test, next key;
+// stores the atom into slot i of B.  vals = (vx^2, vy^2, vz^2, U) of a
+// survivor; lost_e = energy of a removed atom.
+__device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev &tr, MoveAtom &m,
+                                            long long step, int i, const FusedPtrs &B,
+                                            uint32_t *__restrict__ key_out,
+                                            float *__restrict__ evap_pos,
+                                            float *__restrict__ evap_vel, double (&vals)[4],
+                                            double &lost_e, bool &lost, bool &moved)
+{
+    const float dt = mp.f_dt;
+    const float hdt = 0.5f * dt;
+    F3 a0 = accel_f(tr, m.x, m.y, m.z);
+    kahan_add(m.vx, m.cvx, a0.x * hdt);
+    kahan_add(m.vy, m.cvy, a0.y * hdt);
+    kahan_add(m.vz, m.cvz, a0.z * hdt);
+
+    if (mp.spin) {
+        const double dts = mp.dt / mp.substeps;
+        const float w = (float)(mp.gs * kMuB * dts / kHbar);
+        const float fdts = (float)dts;
+        float sx = m.sx, sy = m.sy, sz = m.sz;
+        for (int k = 0; k < mp.substeps; ++k) {
+            const float tau = mp.interp ? ((float)k + 0.5f) * fdts : 0.0f;
+            F3 Bn;
+            const float Bm = field_dir_f(tr, m.x + m.vx * tau, m.y + m.vy * tau, m.z + m.vz * tau, Bn);
+            float sn, cs;
+            sincosf(w * Bm, &sn, &cs);
+            const float nd = Bn.x * sx + Bn.y * sy + Bn.z * sz;
+            const float kx = Bn.y * sz - Bn.z * sy;
+            const float ky = Bn.z * sx - Bn.x * sz;
+            const float kz = Bn.x * sy - Bn.y * sx;
+            const float om = nd * (1.0f - cs);
+            sx = sx * cs + kx * sn + Bn.x * om;
+            sy = sy * cs + ky * sn + Bn.y * om;
+            sz = sz * cs + kz * sn + Bn.z * om;
+        }
+        const double L = sqrt((double)sx * sx + (double)sy * sy + (double)sz * sz);
+        m.sx = (float)((double)sx / L); m.sy = (float)((double)sy / L); m.sz = (float)((double)sz / L);
+    }
+
+    kahan_add(m.x, m.cx, m.vx * dt);
+    kahan_add(m.y, m.cy, m.vy * dt);
+    kahan_add(m.z, m.cz, m.vz * dt);
+    if (tr.periodic) {  // wrap the FP64 value, then split it again
+        double X = (double)m.x + (double)m.cx, Y = (double)m.y + (double)m.cy,
+               Z = (double)m.z + (double)m.cz;
+        if (X >= tr.hw[0] || X < -tr.hw[0]) { X = wrap_periodic(X, tr.hw[0]); split_df(X, m.x, m.cx); }
+        if (Y >= tr.hw[1] || Y < -tr.hw[1]) { Y = wrap_periodic(Y, tr.hw[1]); split_df(Y, m.y, m.cy); }
+        if (Z >= tr.hw[2] || Z < -tr.hw[2]) { Z = wrap_periodic(Z, tr.hw[2]); split_df(Z, m.z, m.cz); }
+    }
+    F3 a1 = accel_f(tr, m.x, m.y, m.z);
+    kahan_add(m.vx, m.cvx, a1.x * hdt);
+    kahan_add(m.vy, m.cvy, a1.y * hdt);
+    kahan_add(m.vz, m.cvz, a1.z * hdt);
+
+    const double X = (double)m.x + (double)m.cx, Y = (double)m.y + (double)m.cy,
+                 Z = (double)m.z + (double)m.cz;
+    const double VX = (double)m.vx + (double)m.cvx, VY = (double)m.vy + (double)m.cvy,
+                 VZ = (double)m.vz + (double)m.cvz;
+    const double ek = 0.5 * mp.mass * (VX * VX + VY * VY + VZ * VZ);
+    const double ep = potential_energy(tr, X, Y, Z);
+    lost = false;
+    if (mp.spin_loss) {
+        F3 Bn;
+        (void)field_dir_f(tr, m.x, m.y, m.z, Bn);
+        lost = Bn.x * m.sx + Bn.y * m.sy + Bn.z * m.sz < 0.0f;
+    }
+    if (mp.energy_cut && tr.kT_cut > 0.0) {
+        const double f = tr.eta_steps > 0 ? fmin((double)step / (double)tr.eta_steps, 1.0) : 1.0;
+        const double eta = tr.eta_start + (tr.eta_end - tr.eta_start) * f;
+        lost = lost || (ek + ep > eta * tr.kT_cut);
+    }
+    uint32_t kout;
+    moved = false;
+    if (lost) {
+        kout = mp.dead_key;
+        const size_t id = (size_t)m.id;
+        evap_pos[3 * id + 0] = (float)X; evap_pos[3 * id + 1] = (float)Y;
+        evap_pos[3 * id + 2] = (float)Z;
+        evap_vel[3 * id + 0] = (float)VX; evap_vel[3 * id + 1] = (float)VY;
+        evap_vel[3 * id + 2] = (float)VZ;
+        lost_e = ek + ep;
+    } else {
+        kout = (uint32_t)(tr.key_base + cell_of_fast(tr, X, Y, Z));
+        vals[0] = VX * VX; vals[1] = VY * VY; vals[2] = VZ * VZ;
+        vals[3] = ep;
+        if (mp.count_moved) moved = kout != m.kold;
+    }
+    st_stream(&B.p[0][i], m.x); st_stream(&B.p[1][i], m.y); st_stream(&B.p[2][i], m.z);
+    st_stream(&B.cp[0][i], m.cx); st_stream(&B.cp[1][i], m.cy); st_stream(&B.cp[2][i], m.cz);
+    st_stream(&B.v[0][i], m.vx); st_stream(&B.v[1][i], m.vy); st_stream(&B.v[2][i], m.vz);
+    st_stream(&B.cv[0][i], m.cvx); st_stream(&B.cv[1][i], m.cvy); st_stream(&B.cv[2][i], m.cvz);
+    if (mp.spin) { st_stream(&B.s[0][i], m.sx); st_stream(&B.s[1][i], m.sy); st_stream(&B.s[2][i], m.sz); }
+    st_stream(&B.id[i], m.id);
+    st_stream(&key_out[i], kout);
+}
+
+// A block of BS threads moves APT * BS consecutive slots, APT per thread: all
+// of a thread's atoms are loaded before the first is updated, so their loads
+// are in flight together.
+template <int BS, int APT>
 __global__ __launch_bounds__(BS) void k_move_fused(
     const TrapDev *__restrict__ traps, MoveParams mp, FusedPtrs A, FusedPtrs B,
     const uint32_t *__restrict__ perm, const int *__restrict__ trap_start,
@@ -533,116 +661,41 @@ __global__ __launch_bounds__(BS) void k_move_fused(
 {
     extern __shared__ int ts[];
     __shared__ double red[(BS / kWave) * 4];
+    __shared__ uint32_t wcnt[BS / kWave][2];
     const int T = mp.n_traps;
     load_table(ts, trap_start, T + 1);
     const int n = *n_dev;
     const long long step = *step_dev;
-    const int i = blockIdx.x * BS + threadIdx.x;
-    const int b0 = blockIdx.x * BS;
+    const int b0 = blockIdx.x * (BS * APT);
     // traps covered by this block: usually one, so no per-thread search
     const int t_first = b0 < n ? last_le(ts, T, b0) : 0;
-    const int t_last = b0 < n ? last_le(ts, T, min(n, b0 + BS) - 1) : 0;
+    const int t_last = b0 < n ? last_le(ts, T, min(n, b0 + BS * APT) - 1) : 0;
+    MoveAtom at[APT];
+#pragma unroll
+    for (int k = 0; k < APT; ++k) {
+        const int i = b0 + k * BS + (int)threadIdx.x;
+        at[k].t = -1;
+        if (i < n) {
+            int t = t_first;
+            while (t < t_last && ts[t + 1] <= i) ++t;
+            at[k].t = t;
+            move_load(mp, A, traps[t], perm ? perm[i] : (uint32_t)i, at[k]);
+        }
+    }
     // per atom: sum v_x^2, v_y^2, v_z^2 and U of survivors (kinetic energy and
     // the alive count follow from them and the trap ranges), and the energy of
     // evaporated atoms
-    double vals[4] = { 0.0, 0.0, 0.0, 0.0 };
-    double lost_e = 0.0;
-    bool lost = false;
-    bool moved = false;
-    int t = -1;
-    if (i < n) {
-        t = t_first;
-        while (t < t_last && ts[t + 1] <= i) ++t;
-        const TrapDev &tr = traps[t];
-        const uint32_t j = perm ? perm[i] : (uint32_t)i;
-        float x = A.p[0][j], y = A.p[1][j], z = A.p[2][j];
-        float cx = A.cp[0][j], cy = A.cp[1][j], cz = A.cp[2][j];
-        float vx = A.v[0][j], vy = A.v[1][j], vz = A.v[2][j];
-        float cvx = A.cv[0][j], cvy = A.cv[1][j], cvz = A.cv[2][j];
-        const int id = A.id[j];
-        const float dt = mp.f_dt;
-        const float hdt = 0.5f * dt;
-
-        F3 a0 = accel_f(tr, x, y, z);
-        kahan_add(vx, cvx, a0.x * hdt);
-        kahan_add(vy, cvy, a0.y * hdt);
-        kahan_add(vz, cvz, a0.z * hdt);
-
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-        if (mp.spin) {
-            sx = A.s[0][j]; sy = A.s[1][j]; sz = A.s[2][j];
-            const double dts = mp.dt / mp.substeps;
-            const float w = (float)(mp.gs * kMuB * dts / kHbar);
-            const float fdts = (float)dts;
-            for (int m = 0; m < mp.substeps; ++m) {
-                const float tau = mp.interp ? ((float)m + 0.5f) * fdts : 0.0f;
-                F3 Bn;
-                const float Bm = field_dir_f(tr, x + vx * tau, y + vy * tau, z + vz * tau, Bn);
-                float sn, cs;
-                sincosf(w * Bm, &sn, &cs);
-                const float nd = Bn.x * sx + Bn.y * sy + Bn.z * sz;
-                const float kx = Bn.y * sz - Bn.z * sy;
-                const float ky = Bn.z * sx - Bn.x * sz;
-                const float kz = Bn.x * sy - Bn.y * sx;
-                const float om = nd * (1.0f - cs);
-                sx = sx * cs + kx * sn + Bn.x * om;
-                sy = sy * cs + ky * sn + Bn.y * om;
-                sz = sz * cs + kz * sn + Bn.z * om;
-            }
-            const double L = sqrt((double)sx * sx + (double)sy * sy + (double)sz * sz);
-            sx = (float)((double)sx / L); sy = (float)((double)sy / L); sz = (float)((double)sz / L);
-        }
-
-        kahan_add(x, cx, vx * dt);
-        kahan_add(y, cy, vy * dt);
-        kahan_add(z, cz, vz * dt);
-        if (tr.periodic) {  // wrap the FP64 value, then split it again
-            double X = (double)x + (double)cx, Y = (double)y + (double)cy, Z = (double)z + (double)cz;
-            if (X >= tr.hw[0] || X < -tr.hw[0]) { X = wrap_periodic(X, tr.hw[0]); split_df(X, x, cx); }
-            if (Y >= tr.hw[1] || Y < -tr.hw[1]) { Y = wrap_periodic(Y, tr.hw[1]); split_df(Y, y, cy); }
-            if (Z >= tr.hw[2] || Z < -tr.hw[2]) { Z = wrap_periodic(Z, tr.hw[2]); split_df(Z, z, cz); }
-        }
-        F3 a1 = accel_f(tr, x, y, z);
-        kahan_add(vx, cvx, a1.x * hdt);
-        kahan_add(vy, cvy, a1.y * hdt);
-        kahan_add(vz, cvz, a1.z * hdt);
-
-        const double X = (double)x + (double)cx, Y = (double)y + (double)cy, Z = (double)z + (double)cz;
-        const double VX = (double)vx + (double)cvx, VY = (double)vy + (double)cvy,
-                     VZ = (double)vz + (double)cvz;
-        const double ek = 0.5 * mp.mass * (VX * VX + VY * VY + VZ * VZ);
-        const double ep = potential_energy(tr, X, Y, Z);
-        if (mp.spin_loss) {
-            F3 Bn;
-            (void)field_dir_f(tr, x, y, z, Bn);
-            lost = Bn.x * sx + Bn.y * sy + Bn.z * sz < 0.0f;
-        }
-        if (mp.energy_cut && tr.kT_cut > 0.0) {
-            const double f = tr.eta_steps > 0 ? fmin((double)step / (double)tr.eta_steps, 1.0) : 1.0;
-            const double eta = tr.eta_start + (tr.eta_end - tr.eta_start) * f;
-            lost = lost || (ek + ep > eta * tr.kT_cut);
-        }
-        uint32_t kout;
-        if (lost) {
-            kout = mp.dead_key;
-            evap_pos[3 * (size_t)id + 0] = (float)X; evap_pos[3 * (size_t)id + 1] = (float)Y;
-            evap_pos[3 * (size_t)id + 2] = (float)Z;
-            evap_vel[3 * (size_t)id + 0] = (float)VX; evap_vel[3 * (size_t)id + 1] = (float)VY;
-            evap_vel[3 * (size_t)id + 2] = (float)VZ;
-            lost_e = ek + ep;
-        } else {
-            kout = (uint32_t)(tr.key_base + cell_of_fast(tr, X, Y, Z));
-            vals[0] = VX * VX; vals[1] = VY * VY; vals[2] = VZ * VZ;
-            vals[3] = ep;
-            if (mp.key_old) moved = kout != mp.key_old[i];
-        }
-        st_stream(&B.p[0][i], x); st_stream(&B.p[1][i], y); st_stream(&B.p[2][i], z);
-        st_stream(&B.cp[0][i], cx); st_stream(&B.cp[1][i], cy); st_stream(&B.cp[2][i], cz);
-        st_stream(&B.v[0][i], vx); st_stream(&B.v[1][i], vy); st_stream(&B.v[2][i], vz);
-        st_stream(&B.cv[0][i], cvx); st_stream(&B.cv[1][i], cvy); st_stream(&B.cv[2][i], cvz);
-        if (mp.spin) { st_stream(&B.s[0][i], sx); st_stream(&B.s[1][i], sy); st_stream(&B.s[2][i], sz); }
-        st_stream(&B.id[i], id);
-        st_stream(&key_out[i], kout);
+    double vals[APT][4], lost_e[APT];
+    bool lost[APT], moved[APT];
+#pragma unroll
+    for (int k = 0; k < APT; ++k) {
+        vals[k][0] = vals[k][1] = vals[k][2] = vals[k][3] = 0.0;
+        lost_e[k] = 0.0;
+        lost[k] = moved[k] = false;
+        const int i = b0 + k * BS + (int)threadIdx.x;
+        if (at[k].t >= 0)
+            move_update(mp, traps[at[k].t], at[k], step, i, B, key_out, evap_pos, evap_vel, vals[k],
+                        lost_e[k], lost[k], moved[k]);
     }
 
     // Per-trap block sums.  Atoms are trap-major, so a block covers traps
@@ -651,16 +704,23 @@ __global__ __launch_bounds__(BS) void k_move_fused(
     // later traps go to acc_over by atomics.  Only the block holding a trap's
     // first slot can do that, so each accumulator receives at most one term
     // and the totals do not depend on the order of execution.
-    __shared__ uint32_t wcnt[BS / kWave][2];
-    if (mp.key_old) {   // per-wave counts, published by the barrier below
-        const uint64_t bm = __ballot(moved), bd = __ballot(lost);
+    bool any = false;
+    if (mp.count_moved) {   // per-wave counts, published by the barrier below
+        uint32_t cm = 0, cd = 0;
+#pragma unroll
+        for (int k = 0; k < APT; ++k) {
+            cm += (uint32_t)__popcll(__ballot(moved[k]));
+            cd += (uint32_t)__popcll(__ballot(lost[k]));
+        }
         if ((threadIdx.x & (kWave - 1)) == 0) {
-            wcnt[threadIdx.x / kWave][0] = (uint32_t)__popcll(bm);
-            wcnt[threadIdx.x / kWave][1] = (uint32_t)__popcll(bd);
+            wcnt[threadIdx.x / kWave][0] = cm;
+            wcnt[threadIdx.x / kWave][1] = cd;
         }
     }
-    const bool any_lost = __syncthreads_or(lost);
-    if (mp.key_old && threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < APT; ++k) any = any || lost[k];
+    const bool any_lost = __syncthreads_or(any);
+    if (mp.count_moved && threadIdx.x == 0) {
         uint32_t nm = 0, nd = 0;
         for (int w = 0; w < BS / kWave; ++w) { nm += wcnt[w][0]; nd += wcnt[w][1]; }
         mp.blk_moved[blockIdx.x] = nm;
@@ -670,14 +730,17 @@ __global__ __launch_bounds__(BS) void k_move_fused(
         const int nseg = t_last - t_first + 1;
         for (int sg = 0; sg < nseg; ++sg) {
             const int tt = t_first + sg;
-            double v[4], e[4];
+            double v[4] = { 0.0, 0.0, 0.0, 0.0 }, e[4] = { 0.0, 0.0, 0.0, 0.0 };
 #pragma unroll
-            for (int q = 0; q < 4; ++q) v[q] = (t == tt) ? vals[q] : 0.0;
+            for (int k = 0; k < APT; ++k)
+                if (at[k].t == tt)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) v[q] += vals[k][q];
             block_sum<BS, 4>(v, red);
-            e[0] = e[1] = e[2] = e[3] = 0.0;
             if (any_lost) {
-                e[0] = (t == tt && lost) ? lost_e : 0.0;
-                e[1] = (t == tt && lost) ? 1.0 : 0.0;
+#pragma unroll
+                for (int k = 0; k < APT; ++k)
+                    if (at[k].t == tt && lost[k]) { e[0] += lost_e[k]; e[1] += 1.0; }
                 block_sum<BS, 4>(e, red);
             }
             if (threadIdx.x == 0) {
@@ -902,7 +965,7 @@ double move_fused_bytes_per_atom(int spin)
     return 4.0 + state + state + 4.0;
 }
 
-template <int BS>
+template <int BS, int APT>
 static void move_fused_bs(const TrapDev *traps, const MoveParams &mp, const FusedPtrs &src,
                           const FusedPtrs &dst, const uint32_t *perm, const int *trap_start,
                           const int *n_dev, uint32_t *key_out, float *evap_pos, float *evap_vel,
@@ -910,35 +973,37 @@ static void move_fused_bs(const TrapDev *traps, const MoveParams &mp, const Fuse
                           hipStream_t st)
 {
     const size_t lds = sizeof(int) * (size_t)(mp.n_traps + 1);
-    hipLaunchKernelGGL(k_move_fused<BS>, dim3(fgrid(n_cap, BS)), dim3(BS), lds, st, traps, mp, src,
-                       dst, perm, trap_start, n_dev, key_out, evap_pos, evap_vel, rec.trap, rec.sum,
-                       rec.stride, rec.acc_over, step_dev);
+    hipLaunchKernelGGL((k_move_fused<BS, APT>), dim3(fgrid(n_cap, BS * APT)), dim3(BS), lds, st,
+                       traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos, evap_vel,
+                       rec.trap, rec.sum, rec.stride, rec.acc_over, step_dev);
 }
 
 int obs_record_stride(int n_cap) { return fgrid(n_cap, 64) + 1; }
 
-void launch_move_fused(int block, const TrapDev *traps, const MoveParams &mp, const FusedPtrs &src,
-                       const FusedPtrs &dst, const uint32_t *perm, const int *trap_start,
-                       const int *n_dev, uint32_t *key_out, float *evap_pos, float *evap_vel,
-                       const ObsRecords &rec, const long long *step_dev, int n_cap, hipStream_t st,
-                       Profiler *prof)
+void launch_move_fused(int block, int apt, const TrapDev *traps, const MoveParams &mp,
+                       const FusedPtrs &src, const FusedPtrs &dst, const uint32_t *perm,
+                       const int *trap_start, const int *n_dev, uint32_t *key_out,
+                       float *evap_pos, float *evap_vel, const ObsRecords &rec,
+                       const long long *step_dev, int n_cap, hipStream_t st, Profiler *prof)
 {
     if (prof && prof->enabled)
         prof->begin(st, "move_fused", (double)n_cap * move_fused_bytes_per_atom(mp.spin));
-#define DSMC_MOVE_CASE(BSV)                                                                     \
-    case BSV:                                                                                   \
-        move_fused_bs<BSV>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,     \
-                           evap_vel, rec, step_dev, n_cap, st);                                 \
-        break;
-    switch (block) {
-        DSMC_MOVE_CASE(64)
-        DSMC_MOVE_CASE(128)
-        DSMC_MOVE_CASE(512)
-        DSMC_MOVE_CASE(1024)
-    default:
-        move_fused_bs<256>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
-                           evap_vel, rec, step_dev, n_cap, st);
-        break;
+#define DSMC_MOVE_CASE(BSV, APTV)                                                               \
+    if (block == BSV && apt == APTV) {                                                          \
+        move_fused_bs<BSV, APTV>(traps, mp, src, dst, perm, trap_start, n_dev, key_out,         \
+                                 evap_pos, evap_vel, rec, step_dev, n_cap, st);                 \
+    } else
+    DSMC_MOVE_CASE(64, 1)
+    DSMC_MOVE_CASE(128, 1)
+    DSMC_MOVE_CASE(512, 1)
+    DSMC_MOVE_CASE(1024, 1)
+    DSMC_MOVE_CASE(64, 2)
+    DSMC_MOVE_CASE(128, 2)
+    DSMC_MOVE_CASE(256, 2)
+    DSMC_MOVE_CASE(512, 2)
+    {
+        move_fused_bs<256, 1>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
+                              evap_vel, rec, step_dev, n_cap, st);
     }
 #undef DSMC_MOVE_CASE
     DSMC_HIP(hipGetLastError());

@@ -38,10 +38,10 @@ struct MoveParams {
     int    n_traps;
     int    obs_cap;      // records in the observables ring (per trap)
     uint32_t dead_key;
-    // when key_old is set: per move block, the atoms whose new key differs
-    // from key_old (the previous sort's key of the slot) and the removed atoms,
-    // for the incremental re-sort of the next step
-    const uint32_t *key_old;
+    // when count_moved is set: per move block, the atoms whose new key differs
+    // from their key before the move (the last sort's key of the slot) and the
+    // removed atoms, for the incremental re-sort of the next step
+    int count_moved;
     uint32_t *blk_moved;
     uint32_t *blk_dead;
 };
@@ -79,11 +79,14 @@ struct ObsRecords {
     unsigned int *cnt; // finalize counters per trap                [T]
 };
 int  obs_record_stride(int n_cap);
-void launch_move_fused(int block, const TrapDev *traps, const MoveParams &mp, const FusedPtrs &src,
-                       const FusedPtrs &dst, const uint32_t *perm, const int *trap_start,
-                       const int *n_dev, uint32_t *key_out, float *evap_pos, float *evap_vel,
-                       const ObsRecords &rec, const long long *step_dev, int n_cap, hipStream_t st,
-                       Profiler *prof);
+// block: threads per block (64 .. 1024); apt: atoms per thread (1 or 2).  A
+// block moves block * apt consecutive slots and leaves one observables record
+// and one pair of re-sort counts.
+void launch_move_fused(int block, int apt, const TrapDev *traps, const MoveParams &mp,
+                       const FusedPtrs &src, const FusedPtrs &dst, const uint32_t *perm,
+                       const int *trap_start, const int *n_dev, uint32_t *key_out,
+                       float *evap_pos, float *evap_vel, const ObsRecords &rec,
+                       const long long *step_dev, int n_cap, hipStream_t st, Profiler *prof);
 // When n_next is set (the move followed a sort), the last block also writes
 // the next step's trap ranges (trap_start) and alive count (*n_next): each
 // trap loses its atoms removed in this step, which the next sort puts last.
