@@ -156,6 +156,7 @@ struct dsmc_ctx {
     FusedPtrs fs[2];
     uint32_t *cell_start = nullptr, *cell_end = nullptr;
     uint32_t *blk_moved = nullptr, *blk_dead = nullptr;   // per move block, for the re-sort
+    uint64_t *mask_moved = nullptr, *mask_dead = nullptr; // per 64 slots, for the re-sort
     int *trap_start = nullptr;
     double *acc = nullptr, *acc2 = nullptr;
     ObsRecords rec{};
@@ -384,6 +385,8 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
         // tiles (never-written entries stay zero)
         x->blk_moved = x->alloc<uint32_t>((size_t)div_up(N, 2048) * (2048 / 64) + 64);
         x->blk_dead = x->alloc<uint32_t>((size_t)div_up(N, 2048) * (2048 / 64) + 64);
+        x->mask_moved = x->alloc<uint64_t>(resort_mask_words(N));
+        x->mask_dead = x->alloc<uint64_t>(resort_mask_words(N));
         x->srt_k[0] = x->key_b; x->srt_v[0] = x->val_b;
         x->srt_k[1] = x->alloc<uint32_t>(N); x->srt_v[1] = x->alloc<uint32_t>(N);
         x->cell_start = x->alloc<uint32_t>(x->n_keys, false);
@@ -551,6 +554,8 @@ static MoveParams move_params(const dsmc_ctx *x)
     mp.count_moved = 0;
     mp.blk_moved = x->blk_moved;
     mp.blk_dead = x->blk_dead;
+    mp.mask_moved = x->mask_moved;
+    mp.mask_dead = x->mask_dead;
     return mp;
 }
 
@@ -571,7 +576,8 @@ static void fused_sort(dsmc_ctx *x)
     uint32_t *ok = x->srt_k[x->srt_cur ^ 1], *ov = x->srt_v[x->srt_cur ^ 1];
     if (x->srt_valid && x->cfg.sort_mode == DSMC_SORT_INCREMENTAL && resort_supported(x->n_cap))
         resort_incremental(x->key_a, x->srt_k[x->srt_cur], n, x->dead_key, x->blk_moved,
-                           x->blk_dead, x->block * x->apt, x->res, ok, ov, x->stream, &x->prof);
+                           x->blk_dead, x->mask_moved, x->mask_dead, x->block * x->apt, x->res,
+                           ok, ov, x->stream, &x->prof);
     else
         radix_sort(x->key_a, nullptr, x->res.alt_k, x->res.alt_v, ok, ov, n, x->key_bits,
                    x->sort, x->stream, &x->prof);

@@ -328,6 +328,18 @@ __device__ __forceinline__ F3 accel_f(const TrapDev &t, float x, float y, float 
 template <typename T>
 __device__ __forceinline__ void st_stream(T *p, T v) { __builtin_nontemporal_store(v, p); }
 
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) round-robin:
+// block b runs on XCD b % 8.  This bijection of [0, nb) gives each XCD one
+// contiguous range of logical blocks, so neighbouring tiles (whose cache
+// lines an out-of-place gather touches) share an L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    const int x = b % kXcds, j = b / kXcds;
+    const int per = nb / kXcds, rem = nb % kXcds;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + j;
+}
+
 // ---------------------------------------------------------------------------
 // Wave64 helpers.
 // ---------------------------------------------------------------------------

@@ -94,7 +94,7 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 // out (which must not alias key_old).  dead_key sorts last.
 struct ResortScratch {
     int n_cap = 0, tiles = 0;
-    uint32_t *stay_k = nullptr, *stay_v = nullptr, *mov_k = nullptr, *mov_v = nullptr;
+    uint32_t *mov_k = nullptr, *mov_v = nullptr;
     uint32_t *mov_q = nullptr, *mov_r = nullptr, *bk_k = nullptr, *bk_v = nullptr, *bk_q = nullptr;
     uint32_t *srt_k = nullptr, *srt_v = nullptr, *srt_q = nullptr;
     uint32_t *alt_k = nullptr, *alt_v = nullptr, *spare = nullptr;
@@ -104,11 +104,14 @@ struct ResortScratch {
 size_t resort_scratch_bytes(int n_cap);
 void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem);
 bool resort_supported(int n_cap);
+// 64-bit words of each of the mover and removal masks for n_cap slots.
+size_t resort_mask_words(int n_cap);
 // blk_moved / blk_dead: per block of move_block slots, the atoms whose key
-// changed (not removed) and the removed atoms (written by the fused move).
+// changed (not removed) and the removed atoms; mask_m / mask_d: the same
+// atoms as one bit per slot (written by the fused move).
 void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const int *n_dev,
                         uint32_t dead_key, const uint32_t *blk_moved, const uint32_t *blk_dead,
-                        int move_block, const ResortScratch &r, uint32_t *out_k, uint32_t *out_v,
+                        const uint64_t *mask_m, const uint64_t *mask_d, int move_block, const ResortScratch &r, uint32_t *out_k, uint32_t *out_v,
                         hipStream_t stream, Profiler *prof);
 
 // Exclusive scan of n ints (n known on the host); partial holds div_up(n, tile).

@@ -705,12 +705,18 @@ __global__ __launch_bounds__(BS) void k_move_fused(
     // first slot can do that, so each accumulator receives at most one term
     // and the totals do not depend on the order of execution.
     bool any = false;
-    if (mp.count_moved) {   // per-wave counts, published by the barrier below
+    if (mp.count_moved) {   // per-wave masks and counts (published by the barrier below)
         uint32_t cm = 0, cd = 0;
 #pragma unroll
         for (int k = 0; k < APT; ++k) {
-            cm += (uint32_t)__popcll(__ballot(moved[k]));
-            cd += (uint32_t)__popcll(__ballot(lost[k]));
+            const uint64_t bm = __ballot(moved[k]), bd = __ballot(lost[k]);
+            cm += (uint32_t)__popcll(bm);
+            cd += (uint32_t)__popcll(bd);
+            if ((threadIdx.x & (kWave - 1)) == 0) {
+                const int wd = (b0 + k * BS) / kWave + (int)threadIdx.x / kWave;
+                mp.mask_moved[wd] = bm;
+                mp.mask_dead[wd] = bd;
+            }
         }
         if ((threadIdx.x & (kWave - 1)) == 0) {
             wcnt[threadIdx.x / kWave][0] = cm;

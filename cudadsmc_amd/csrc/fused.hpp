@@ -40,10 +40,13 @@ struct MoveParams {
     uint32_t dead_key;
     // when count_moved is set: per move block, the atoms whose new key differs
     // from their key before the move (the last sort's key of the slot) and the
-    // removed atoms, for the incremental re-sort of the next step
+    // removed atoms, and the same atoms as one bit per slot (64-slot words),
+    // for the incremental re-sort of the next step
     int count_moved;
     uint32_t *blk_moved;
     uint32_t *blk_dead;
+    uint64_t *mask_moved;
+    uint64_t *mask_dead;
 };
 
 struct TableParams {

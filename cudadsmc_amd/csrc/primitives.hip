@@ -198,35 +198,45 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 
 // ---------------------------------------------------------------------------
 // Incremental re-sort.  Input: keys key_new[0..n) of slots that were in
-// stable sorted order of key_old[0..n) (the previous sort's keys).  Output:
-// the stable sort of key_new by (key, slot) -- the same permutation a full
-// radix sort gives -- in a buffer other than key_old.  Between time steps
-// only the atoms that crossed a cell face change key (a few percent), so
-// instead of sorting:
-//   1. split: slots whose key did not change ("stays": already in (key, slot)
-//      order), live "movers" and removed atoms (dead key, which sorts last:
-//      written straight to the end of the output, in slot order);
-//   2. locate: each mover's insertion point q = number of stays that precede
-//      it (binary search); movers are bucketed by the 2048-stay tile of q;
-//   3. place: movers are copied into their bucket;
-//   4. merge: one block per stay tile ranks its bucket's movers by counting
-//      (buckets hold a few dozen), and writes stays and movers to their final
-//      positions: stay g goes to g + (movers before it), a mover of rank r in
-//      the tile's bucket to q + (movers in earlier buckets) + r.
+// stable sorted order of key_old[0..n) (the previous sort's keys), and the
+// move kernel's per-64-slot masks of the slots whose key changed ("movers")
+// and of removed atoms (dead key, which sorts last).  Output: the stable sort
+// of key_new by (key, slot) -- the same permutation a full radix sort gives
+// -- in a buffer other than key_old.  Between time steps only the atoms that
+// crossed a cell face change key (a few percent); the others ("stays") keep
+// their order, so nothing is sorted but the movers, and the stays are never
+// copied out of slot order:
+//   1. tiles: movers and removed atoms before every 2048-slot tile
+//      (prefix sums of the move kernel's per-block counts);
+//   2. collect: the movers' (key, slot) in slot order; removed atoms are
+//      written straight to the end of the output, in slot order;
+//   3. locate: each mover's insertion point q in slot space, the first slot
+//      j with (key_old[j], j) >= (key, slot) (binary search of key_old, whose
+//      stays equal key_new): the mover goes after every stay below q and
+//      before every stay from q on.  Movers are bucketed by the tile of q;
+//   4. place: movers are copied into their bucket;
+//   5. merge: one block per tile ranks its bucket's movers by counting
+//      (buckets hold a few dozen), and writes the tile's stays and movers to
+//      their final positions: with S = stays before the tile plus movers in
+//      earlier buckets, a stay goes to S + (stays of the tile before it) +
+//      (bucket movers with q <= its slot), a mover of rank r to S + (stays
+//      of the tile below q) + r.
 // No kernel waits on another block.
 // ---------------------------------------------------------------------------
-constexpr int kSplitTile = kSortTile;          // slots per split block (256 x 8)
-constexpr int kStayTile = 2048;                // stays per bucket
+constexpr int kStayTile = 2048;                // slots per tile (and bucket)
 constexpr int kStayShift = 11;
+constexpr int kTileWords = kStayTile / 64;     // mask words per tile
 constexpr int kBucketLds = 1024;               // movers a bucket ranks in LDS
 constexpr int kBucketBits = 13;
 constexpr int kMaxBuckets = 1 << kBucketBits;  // LDS bucket table of the place kernel
 static_assert((1 << kStayShift) == kStayTile, "stay tile");
+static_assert(kTileWords == 32, "a tile's mask words are half a wave");
 
-// Per split tile: movers and removed atoms before it (exclusive prefixes of
-// the counts the move kernel left per move block) and the removed total.
-// Block k handles tiles [256 k, 256 k + 256) and adds up the earlier tiles
-// itself (no block waits for another).
+// Per tile: movers and removed atoms before it (exclusive prefixes of the
+// counts the move kernel left per move block); entry `tiles` holds the
+// totals, and counts[] = {stays, movers, removed}.  Block k handles tiles
+// [256 k, 256 k + 256) and adds up the earlier tiles itself (no block waits
+// for another).
 constexpr int kTilesBlock = 256;
 
 __global__ __launch_bounds__(kTilesBlock) void k_ins_tiles(const uint32_t *__restrict__ blk_moved,
@@ -238,7 +248,7 @@ __global__ __launch_bounds__(kTilesBlock) void k_ins_tiles(const uint32_t *__res
 {
     __shared__ uint32_t tmp[kTilesBlock / kWave + 1];
     const int n = *n_dev;
-    const int tiles = n > 0 ? (n - 1) / kSplitTile + 1 : 1;
+    const int tiles = n > 0 ? (n - 1) / kStayTile + 1 : 1;
     const int t0 = blockIdx.x * kTilesBlock;
     if (t0 >= tiles) return;   // whole block leaves together
     // earlier tiles: their move blocks [0, t0 * per_tile), a multiple of 1024
@@ -280,82 +290,63 @@ __global__ __launch_bounds__(kTilesBlock) void k_ins_tiles(const uint32_t *__res
         pre_m[t] = bm + em;
         pre_d[t] = bd + ed;
     }
-    if (t0 + kTilesBlock >= tiles && threadIdx.x == 0) counts[3] = (int)(bd + td);
+    if (t0 + kTilesBlock >= tiles && threadIdx.x == 0) {
+        const uint32_t M = bm + tm, D = bd + td;
+        pre_m[tiles] = M;
+        pre_d[tiles] = D;
+        counts[0] = n - (int)M - (int)D;
+        counts[1] = (int)M;
+        counts[2] = (int)D;
+    }
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_ins_split(
-    const uint32_t *__restrict__ key_new, const uint32_t *__restrict__ key_old,
-    const int *__restrict__ n_dev, uint32_t dead_key, int tiles,
-    const uint32_t *__restrict__ pre_m, const uint32_t *__restrict__ pre_d,
-    uint32_t *__restrict__ stay_k, uint32_t *__restrict__ stay_v, uint32_t *__restrict__ mov_k,
-    uint32_t *__restrict__ mov_v, uint32_t *__restrict__ mov_p, uint32_t *__restrict__ out_k,
-    uint32_t *__restrict__ out_v, int *__restrict__ counts, uint32_t *__restrict__ bcnt)
+// Exclusive prefix of v over the 32 lanes of this lane's half-wave.
+__device__ __forceinline__ uint32_t halfwave_exclusive_scan(uint32_t v)
 {
-    constexpr int W = kSortBlock / kWave;
-    __shared__ uint32_t wcnt[2][W];
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
-    // clear the bucket counts for the locate step (tiles + 2 entries)
-    if (tid == 0) bcnt[blockIdx.x] = 0u;
-    if (blockIdx.x == 0 && tid < 2) bcnt[tiles + tid] = 0u;
+    const int hl = threadIdx.x & 31;
+    uint32_t s = v;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+        const uint32_t u = __shfl_up(s, d, 32);
+        if (hl >= d) s += u;
+    }
+    return s - v;
+}
+
+// One thread per 64-slot mask word (a tile's 32 words are half a wave):
+// writes the word's movers (key, slot) in slot order to mov_k / mov_v and its
+// removed atoms to the end of the output.  Also clears the bucket counts.
+__global__ __launch_bounds__(256) void k_ins_collect(
+    const uint32_t *__restrict__ key_new, const uint64_t *__restrict__ mask_m,
+    const uint64_t *__restrict__ mask_d, const int *__restrict__ n_dev, uint32_t dead_key,
+    const uint32_t *__restrict__ pre_m, const uint32_t *__restrict__ pre_d,
+    const int *__restrict__ counts, uint32_t *__restrict__ mov_k, uint32_t *__restrict__ mov_v,
+    uint32_t *__restrict__ out_k, uint32_t *__restrict__ out_v, uint32_t *__restrict__ bcnt)
+{
     const int n = *n_dev;
-    const int tile0 = blockIdx.x * kSplitTile;
-    const int last_tile = n > 0 ? (n - 1) / kSplitTile : 0;
-    if ((int)blockIdx.x > last_tile) return;  // whole block leaves together
-    // movers and removed atoms in earlier tiles, removed atoms in all tiles
-    const uint32_t mb = pre_m[blockIdx.x], db = pre_d[blockIdx.x], dall = (uint32_t)counts[3];
-    // each wave owns a contiguous run of kWave * kSortItems slots
-    const int base = tile0 + w * (kWave * kSortItems);
-    const uint64_t lt = lanemask_lt();
-    uint32_t kn[kSortItems];
-    uint8_t kind[kSortItems];    // 0 stay, 1 mover, 2 removed
-    uint32_t rm[kSortItems], rd[kSortItems];
-    uint32_t runm = 0, rund = 0;
-#pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
-        const int i = base + k * kWave + lane;
-        const bool valid = i < n;
-        kn[k] = valid ? key_new[i] : 0u;
-        const bool dead = valid && kn[k] == dead_key;
-        const bool mov = valid && !dead && kn[k] != key_old[i];
-        kind[k] = dead ? 2 : (mov ? 1 : 0);
-        const uint64_t bm = __ballot(mov), bd = __ballot(dead);
-        rm[k] = runm + (uint32_t)__popcll(bm & lt);
-        rd[k] = rund + (uint32_t)__popcll(bd & lt);
-        runm += (uint32_t)__popcll(bm);
-        rund += (uint32_t)__popcll(bd);
+    const int g = blockIdx.x * 256 + (int)threadIdx.x;
+    if (g <= (n >> kStayShift)) bcnt[g] = 0u;   // buckets 0 .. n / 2048
+    const int words = (n + 63) >> 6;
+    if ((int)blockIdx.x * 256 >= words) return;   // whole block leaves together
+    const bool valid = g < words;
+    const uint64_t mm = valid ? mask_m[g] : 0ull, md = valid ? mask_d[g] : 0ull;
+    const uint32_t em = halfwave_exclusive_scan((uint32_t)__popcll(mm));
+    const uint32_t ed = halfwave_exclusive_scan((uint32_t)__popcll(md));
+    if (!valid) return;
+    const int tile = g >> 5;
+    uint32_t o = pre_m[tile] + em;
+    for (uint64_t b = mm; b; b &= b - 1) {
+        const uint32_t slot = (uint32_t)g * 64u + (uint32_t)(__ffsll((unsigned long long)b) - 1);
+        mov_k[o] = key_new[slot];
+        mov_v[o] = slot;
+        ++o;
     }
-    if (lane == 0) { wcnt[0][w] = runm; wcnt[1][w] = rund; }
-    __syncthreads();
-    uint32_t wm = 0, wd = 0, bm_all = 0;
-    for (int ww = 0; ww < W; ++ww) {
-        if (ww < w) { wm += wcnt[0][ww]; wd += wcnt[1][ww]; }
-        bm_all += wcnt[0][ww];
-    }
-#pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
-        const int i = base + k * kWave + lane;
-        if (i >= n) continue;
-        const uint32_t mbefore = mb + wm + rm[k];   // movers before slot i
-        const uint32_t dbefore = db + wd + rd[k];   // removed atoms before slot i
-        if (kind[k] == 1) {
-            mov_k[mbefore] = kn[k];
-            mov_v[mbefore] = (uint32_t)i;
-            mov_p[mbefore] = (uint32_t)i - mbefore - dbefore;   // stays with a smaller slot
-        } else if (kind[k] == 2) {
-            const uint32_t o = (uint32_t)n - dall + dbefore;
-            out_k[o] = kn[k];
-            out_v[o] = (uint32_t)i;
-        } else {
-            const uint32_t sp = (uint32_t)i - mbefore - dbefore;
-            stay_k[sp] = kn[k];
-            stay_v[sp] = (uint32_t)i;
-        }
-    }
-    if ((int)blockIdx.x == last_tile && tid == 0) {
-        const int movers = (int)(mb + bm_all);
-        counts[0] = n - movers - (int)dall;
-        counts[1] = movers;
-        counts[2] = (int)dall;
+    o = (uint32_t)(n - counts[2]) + pre_d[tile] + ed;
+    for (uint64_t b = md; b; b &= b - 1) {
+        const uint32_t slot = (uint32_t)g * 64u + (uint32_t)(__ffsll((unsigned long long)b) - 1);
+        out_k[o] = dead_key;
+        out_v[o] = slot;
+        ++o;
     }
 }
 
@@ -365,41 +356,40 @@ __device__ __forceinline__ bool kv_less(uint32_t ka, uint32_t va, uint32_t kb, u
     return ka < kb || (ka == kb && va < vb);
 }
 
-// Insertion point q of every mover among the stays (the number of stays that
-// precede it), and its slot in q's bucket.  The search gallops from the
-// number of stays with a smaller slot (where the mover sat before its key
-// changed): a move to the next cell along x lands within a few places,
-// along y or z within about Nc or Nc^2 cells' worth of atoms.  mov_r holds
-// that start on entry and the bucket slot on exit.
+// Insertion point q of every mover in slot space: the first slot j with
+// (key_old[j], j) >= (key, slot) (key_old is sorted, so this is a prefix).
+// The search gallops from the mover's own slot: a move to the next cell
+// along x lands within a few places, along y or z within about Nc or Nc^2
+// cells' worth of atoms.  Also the mover's place in q's bucket.
 __global__ __launch_bounds__(256) void k_ins_locate(
-    const uint32_t *__restrict__ stay_k, const uint32_t *__restrict__ stay_v,
-    const uint32_t *__restrict__ mov_k, const uint32_t *__restrict__ mov_v,
+    const uint32_t *__restrict__ key_old, const uint32_t *__restrict__ mov_k,
+    const uint32_t *__restrict__ mov_v, const int *__restrict__ n_dev,
     const int *__restrict__ counts, uint32_t *__restrict__ mov_q, uint32_t *__restrict__ mov_r,
     uint32_t *__restrict__ bcnt)
 {
     const int j = blockIdx.x * 256 + threadIdx.x;
-    const int s = counts[0], m = counts[1];
+    const int s = *n_dev, m = counts[1];
     if (blockIdx.x * 256 >= m) return;   // whole block leaves together
     const bool valid = j < m;
     const uint32_t k = valid ? mov_k[j] : 0u, v = valid ? mov_v[j] : 0u;
-    const int p0 = valid ? (int)mov_r[j] : 0;
+    const int p0 = (int)v;
     int lo, hi;   // the insertion point lies in [lo, hi]
     if (!valid) {
         lo = hi = 0;
-    } else if (p0 < s && kv_less(stay_k[p0], stay_v[p0], k, v)) {
+    } else if (key_old[p0] < k) {   // (key_old[p0], p0) < (k, v): key_old[p0] != k
         int off = 1;
-        while (p0 + off < s && kv_less(stay_k[p0 + off], stay_v[p0 + off], k, v)) off <<= 1;
+        while (p0 + off < s && kv_less(key_old[p0 + off], (uint32_t)(p0 + off), k, v)) off <<= 1;
         lo = p0 + (off >> 1) + 1;
         hi = min(p0 + off, s);
     } else {
         int off = 1;
-        while (p0 - off >= 0 && !kv_less(stay_k[p0 - off], stay_v[p0 - off], k, v)) off <<= 1;
+        while (p0 - off >= 0 && !kv_less(key_old[p0 - off], (uint32_t)(p0 - off), k, v)) off <<= 1;
         lo = p0 - off < 0 ? 0 : p0 - off + 1;
-        hi = off == 1 ? p0 : p0 - (off >> 1);
+        hi = p0 - (off >> 1);
     }
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (kv_less(stay_k[mid], stay_v[mid], k, v)) lo = mid + 1; else hi = mid;
+        if (kv_less(key_old[mid], (uint32_t)mid, k, v)) lo = mid + 1; else hi = mid;
     }
     // bucket slot: lanes of a wave with the same bucket share one atomic
     // (nearby movers mostly share buckets)
@@ -435,14 +425,15 @@ constexpr int kPlaceBlocks = 256;
 __global__ __launch_bounds__(256) void k_ins_place(
     const uint32_t *__restrict__ mov_k, const uint32_t *__restrict__ mov_v,
     const uint32_t *__restrict__ mov_q, const uint32_t *__restrict__ mov_r,
-    const int *__restrict__ counts, const uint32_t *__restrict__ bcnt,
+    const int *__restrict__ n_dev, const int *__restrict__ counts,
+    const uint32_t *__restrict__ bcnt,
     uint32_t *__restrict__ bk_k, uint32_t *__restrict__ bk_v, uint32_t *__restrict__ bk_q,
     uint32_t *__restrict__ bex)
 {
     __shared__ uint32_t ex[kMaxBuckets];
     __shared__ uint32_t tmp[256 / kWave + 1];
-    const int s = counts[0], m = counts[1];
-    const int nbk = (s >> kStayShift) + 1;
+    const int m = counts[1];
+    const int nbk = (*n_dev >> kStayShift) + 1;
     bucket_scan(bcnt, nbk, ex, tmp);
     if (blockIdx.x == 0)   // bucket offsets for the merge
         for (int i = threadIdx.x; i < nbk; i += 256) bex[i] = ex[i];
@@ -525,30 +516,50 @@ __device__ int big_bucket_sort(uint32_t *k0, uint32_t *v0, uint32_t *q0, uint32_
     return src;
 }
 
-// One block per stay tile t: rank the tile's bucket, then write stays and
-// movers to out[] at their final positions (a contiguous window).  Buckets
-// larger than the LDS table are sorted in global scratch by the block.
+// One block per tile t: rank the tile's bucket, then write the tile's stays
+// and movers to out[] at their final positions (a contiguous window).
+// Buckets larger than the LDS table are sorted in global scratch by the block.
 __global__ __launch_bounds__(256) void k_ins_merge(
-    const uint32_t *__restrict__ stay_k, const uint32_t *__restrict__ stay_v,
+    const uint32_t *__restrict__ key_new, const uint64_t *__restrict__ mask_m,
+    const uint64_t *__restrict__ mask_d, const int *__restrict__ n_dev,
+    const uint32_t *__restrict__ pre_m, const uint32_t *__restrict__ pre_d,
     const uint32_t *__restrict__ bk_k, const uint32_t *__restrict__ bk_v,
-    const uint32_t *__restrict__ bk_q, const int *__restrict__ counts,
-    const uint32_t *__restrict__ bcnt, const uint32_t *__restrict__ bex,
-    uint32_t *__restrict__ srt_k, uint32_t *__restrict__ srt_v,
+    const uint32_t *__restrict__ bk_q, const uint32_t *__restrict__ bcnt,
+    const uint32_t *__restrict__ bex, uint32_t *__restrict__ srt_k, uint32_t *__restrict__ srt_v,
     uint32_t *__restrict__ srt_q, uint32_t *__restrict__ alt_k, uint32_t *__restrict__ alt_v,
     uint32_t *__restrict__ alt_q, uint32_t *__restrict__ out_k, uint32_t *__restrict__ out_v)
 {
     __shared__ uint32_t lk[kBucketLds], lv[kBucketLds], lq[kBucketLds];
     __shared__ uint32_t sk[kBucketLds], sv[kBucketLds], sq[kBucketLds];
-    const int s = counts[0];
+    __shared__ uint64_t stw[kTileWords];   // stay bits of the tile's words
+    __shared__ uint32_t wst[kTileWords];   // stays of the tile before each word
+    const int n = *n_dev;
     const int t = blockIdx.x;
-    if (t > (s >> kStayShift)) return;   // whole block leaves together
+    if (t > (n >> kStayShift)) return;   // whole block leaves together
     const int tid = threadIdx.x;
+    const int g0 = t * kStayTile;
+    const int ns = min(kStayTile, n - g0);   // slots of the tile (0 for the bucket at n)
+    if (tid < kTileWords) {
+        const int l0 = tid * 64;
+        uint64_t st = 0ull;
+        if (l0 < ns) {
+            const int w = (g0 >> 6) + tid;
+            const uint64_t in = ns - l0 >= 64 ? ~0ull : (1ull << (ns - l0)) - 1ull;
+            st = ~(mask_m[w] | mask_d[w]) & in;
+        }
+        stw[tid] = st;
+        wst[tid] = halfwave_exclusive_scan((uint32_t)__popcll(st));
+    }
     const uint32_t before = bex[t];   // movers in earlier buckets
     const int b = (int)bcnt[t];
-    const int g0 = t * kStayTile;
-    const int ns = min(kStayTile, s - g0);
-    const uint32_t w0 = (uint32_t)g0 + before;   // first output slot of this tile
+    // first output slot of the tile: stays of earlier tiles + earlier movers
+    const uint32_t w0 = (uint32_t)g0 - pre_m[t] - pre_d[t] + before;
     const uint32_t *bk = bk_k + before, *bv = bk_v + before, *bq = bk_q + before;
+    // stays of the tile in slots below g0 + li (li < kStayTile)
+    auto stays_below = [&](int li) -> uint32_t {
+        const uint64_t below = (li & 63) ? stw[li >> 6] & (~0ull >> (64 - (li & 63))) : 0ull;
+        return wst[li >> 6] + (uint32_t)__popcll(below);
+    };
     const uint32_t *rq;   // bucket q values in rank order
     if (b <= kBucketLds) {
         for (int e = tid; e < b; e += 256) { lk[e] = bk[e]; lv[e] = bv[e]; lq[e] = bq[e]; }
@@ -562,7 +573,7 @@ __global__ __launch_bounds__(256) void k_ins_merge(
         __syncthreads();
         rq = sq;
         for (int r = tid; r < b; r += 256) {
-            const uint32_t o = w0 + (sq[r] - (uint32_t)g0) + (uint32_t)r;
+            const uint32_t o = w0 + stays_below((int)sq[r] - g0) + (uint32_t)r;
             out_k[o] = sk[r];
             out_v[o] = sv[r];
         }
@@ -576,30 +587,31 @@ __global__ __launch_bounds__(256) void k_ins_merge(
         const uint32_t *gk = which ? ck : ak, *gv = which ? cv : av, *gq = which ? cq : aq;
         rq = gq;
         for (int r = tid; r < b; r += 256) {
-            const uint32_t o = w0 + (gq[r] - (uint32_t)g0) + (uint32_t)r;
+            const uint32_t o = w0 + stays_below((int)gq[r] - g0) + (uint32_t)r;
             out_k[o] = gk[r];
             out_v[o] = gv[r];
         }
     }
-    // stay g0 + li moves up by the bucket movers inserted at or before it
+    // a stay moves up by the bucket movers inserted at or before its slot
     for (int li = tid; li < ns; li += 256) {
+        if (!((stw[li >> 6] >> (li & 63)) & 1ull)) continue;
         const uint32_t g = (uint32_t)(g0 + li);
         int lo = 0, hi = b;   // first rank whose q exceeds g
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (rq[mid] <= g) lo = mid + 1; else hi = mid;
         }
-        const uint32_t o = w0 + (uint32_t)li + (uint32_t)lo;
-        out_k[o] = stay_k[g];
-        out_v[o] = stay_v[g];
+        const uint32_t o = w0 + stays_below(li) + (uint32_t)lo;
+        out_k[o] = key_new[g];
+        out_v[o] = g;
     }
 }
 
 size_t resort_scratch_bytes(int n_cap)
 {
     const size_t n = (size_t)(n_cap > 0 ? n_cap : 1);
-    const size_t tiles = (size_t)div_up((long long)n, kSplitTile) + 2;
-    return 15 * n * sizeof(uint32_t) + 5 * tiles * sizeof(uint32_t) + 8 * sizeof(int) + 24 * 256;
+    const size_t tiles = (size_t)div_up((long long)n, kStayTile) + 2;
+    return 13 * n * sizeof(uint32_t) + 4 * tiles * sizeof(uint32_t) + 8 * sizeof(int) + 24 * 256;
 }
 
 void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem)
@@ -607,11 +619,10 @@ void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem)
     auto align = [](char *p) { return (char *)(((uintptr_t)p + 255) & ~(uintptr_t)255); };
     const size_t n = (size_t)(n_cap > 0 ? n_cap : 1);
     r.n_cap = (int)n;
-    r.tiles = div_up((long long)n, kSplitTile);
+    r.tiles = div_up((long long)n, kStayTile);
     char *p = align((char *)mem);
-    uint32_t **arrays[15] = {&r.stay_k, &r.stay_v, &r.mov_k, &r.mov_v, &r.mov_q, &r.mov_r,
-                             &r.bk_k,   &r.bk_v,   &r.bk_q,  &r.srt_k, &r.srt_v, &r.srt_q,
-                             &r.alt_k,  &r.alt_v,  &r.spare};
+    uint32_t **arrays[13] = {&r.mov_k, &r.mov_v, &r.mov_q, &r.mov_r, &r.bk_k,  &r.bk_v, &r.bk_q,
+                             &r.srt_k, &r.srt_v, &r.srt_q, &r.alt_k, &r.alt_v, &r.spare};
     for (auto a : arrays) { *a = (uint32_t *)p; p = align(p + n * sizeof(uint32_t)); }
     r.tile_cnt = (uint32_t *)p; p = align(p + (size_t)(r.tiles + 2) * sizeof(uint32_t));
     r.tile_dead = (uint32_t *)p; p = align(p + (size_t)(r.tiles + 2) * sizeof(uint32_t));
@@ -622,29 +633,39 @@ void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem)
 
 bool resort_supported(int n_cap)
 {
-    // the place kernel keeps one LDS entry per stay tile
-    return div_up(n_cap > 0 ? n_cap : 1, kStayTile) + 1 <= kMaxBuckets;
+    // the place kernel keeps one LDS entry per tile (n / 2048 + 1 buckets)
+    return (n_cap > 0 ? n_cap : 1) / kStayTile + 1 <= kMaxBuckets;
+}
+
+size_t resort_mask_words(int n_cap)
+{
+    // whole tiles, so that every word of a move grid of blocks up to 2048
+    // slots is inside
+    return (size_t)div_up(n_cap > 0 ? n_cap : 1, kStayTile) * kTileWords + kTileWords;
 }
 
 void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const int *n_dev,
                         uint32_t dead_key, const uint32_t *blk_moved, const uint32_t *blk_dead,
-                        int move_block, const ResortScratch &r, uint32_t *out_k, uint32_t *out_v,
+                        const uint64_t *mask_m, const uint64_t *mask_d, int move_block,
+                        const ResortScratch &r, uint32_t *out_k, uint32_t *out_v,
                         hipStream_t stream, Profiler *prof)
 {
     const double n = (double)r.n_cap;
+    const int words = div_up(r.n_cap, 64) + 2;
     DSMC_LAUNCH(prof, "resort_tiles", 0.0, k_ins_tiles, div_up(r.tiles, kTilesBlock), kTilesBlock,
-                stream, blk_moved, blk_dead, kSplitTile / move_block, n_dev, r.tile_cnt,
+                stream, blk_moved, blk_dead, kStayTile / move_block, n_dev, r.tile_cnt,
                 r.tile_dead, r.counts);
-    DSMC_LAUNCH(prof, "resort_split", n * 16.0, k_ins_split, r.tiles, kSortBlock, stream,
-                key_new, key_old, n_dev, dead_key, r.tiles, r.tile_cnt, r.tile_dead, r.stay_k,
-                r.stay_v, r.mov_k, r.mov_v, r.mov_r, out_k, out_v, r.counts, r.bcnt);
+    DSMC_LAUNCH(prof, "resort_collect", 0.0, k_ins_collect, div_up(words, 256), 256, stream,
+                key_new, mask_m, mask_d, n_dev, dead_key, r.tile_cnt, r.tile_dead, r.counts,
+                r.mov_k, r.mov_v, out_k, out_v, r.bcnt);
     DSMC_LAUNCH(prof, "resort_locate", 0.0, k_ins_locate, div_up(r.n_cap, 256), 256, stream,
-                r.stay_k, r.stay_v, r.mov_k, r.mov_v, r.counts, r.mov_q, r.mov_r, r.bcnt);
+                key_old, r.mov_k, r.mov_v, n_dev, r.counts, r.mov_q, r.mov_r, r.bcnt);
     DSMC_LAUNCH(prof, "resort_place", 0.0, k_ins_place, kPlaceBlocks, 256, stream, r.mov_k,
-                r.mov_v, r.mov_q, r.mov_r, r.counts, r.bcnt, r.bk_k, r.bk_v, r.bk_q, r.bex);
-    DSMC_LAUNCH(prof, "resort_merge", n * 16.0, k_ins_merge, div_up(r.n_cap, kStayTile) + 1, 256,
-                stream, r.stay_k, r.stay_v, r.bk_k, r.bk_v, r.bk_q, r.counts, r.bcnt, r.bex, r.srt_k,
-                r.srt_v, r.srt_q, r.alt_k, r.alt_v, r.spare, out_k, out_v);
+                r.mov_v, r.mov_q, r.mov_r, n_dev, r.counts, r.bcnt, r.bk_k, r.bk_v, r.bk_q, r.bex);
+    DSMC_LAUNCH(prof, "resort_merge", n * 12.0, k_ins_merge, div_up(r.n_cap, kStayTile) + 1, 256,
+                stream, key_new, mask_m, mask_d, n_dev, r.tile_cnt, r.tile_dead, r.bk_k, r.bk_v,
+                r.bk_q, r.bcnt, r.bex, r.srt_k, r.srt_v, r.srt_q, r.alt_k, r.alt_v, r.spare, out_k,
+                out_v);
 }
 
 // ---------------------------------------------------------------------------
