@@ -328,16 +328,20 @@ __device__ __forceinline__ F3 accel_f(const TrapDev &t, float x, float y, float 
 template <typename T>
 __device__ __forceinline__ void st_stream(T *p, T v) { __builtin_nontemporal_store(v, p); }
 
-// Workgroups are dispatched to the 8 XCDs (each with its own L2) round-robin:
-// block b runs on XCD b % 8.  This bijection of [0, nb) gives each XCD one
-// contiguous range of logical blocks, so neighbouring tiles (whose cache
-// lines an out-of-place gather touches) share an L2.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_block(int b, int nb)
+// Element i of an array of at most 2^30 4-byte elements, addressed as the
+// array base plus a 32-bit byte offset: the load or store then takes the base
+// from scalar registers and one offset register (no 64-bit address per array).
+template <typename T>
+__device__ __forceinline__ T ld_at(const T *base, uint32_t i)
 {
-    const int x = b % kXcds, j = b / kXcds;
-    const int per = nb / kXcds, rem = nb % kXcds;
-    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + j;
+    static_assert(sizeof(T) == 4, "4-byte elements");
+    return *(const T *)((const char *)base + (size_t)(i * 4u));
+}
+template <typename T>
+__device__ __forceinline__ void st_stream_at(T *base, uint32_t i, T v)
+{
+    static_assert(sizeof(T) == 4, "4-byte elements");
+    __builtin_nontemporal_store(v, (T *)((char *)base + (size_t)(i * 4u)));
 }
 
 // ---------------------------------------------------------------------------

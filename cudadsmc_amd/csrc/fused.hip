@@ -534,13 +534,13 @@ struct MoveAtom {
 __device__ __forceinline__ void move_load(const MoveParams &mp, const FusedPtrs &A,
                                           const TrapDev &tr, uint32_t j, MoveAtom &m)
 {
-    m.x = A.p[0][j]; m.y = A.p[1][j]; m.z = A.p[2][j];
-    m.cx = A.cp[0][j]; m.cy = A.cp[1][j]; m.cz = A.cp[2][j];
-    m.vx = A.v[0][j]; m.vy = A.v[1][j]; m.vz = A.v[2][j];
-    m.cvx = A.cv[0][j]; m.cvy = A.cv[1][j]; m.cvz = A.cv[2][j];
-    m.id = A.id[j];
+    m.x = ld_at(A.p[0], j); m.y = ld_at(A.p[1], j); m.z = ld_at(A.p[2], j);
+    m.cx = ld_at(A.cp[0], j); m.cy = ld_at(A.cp[1], j); m.cz = ld_at(A.cp[2], j);
+    m.vx = ld_at(A.v[0], j); m.vy = ld_at(A.v[1], j); m.vz = ld_at(A.v[2], j);
+    m.cvx = ld_at(A.cv[0], j); m.cvy = ld_at(A.cv[1], j); m.cvz = ld_at(A.cv[2], j);
+    m.id = ld_at(A.id, j);
     m.sx = m.sy = m.sz = 0.0f;
-    if (mp.spin) { m.sx = A.s[0][j]; m.sy = A.s[1][j]; m.sz = A.s[2][j]; }
+    if (mp.spin) { m.sx = ld_at(A.s[0], j); m.sy = ld_at(A.s[1], j); m.sz = ld_at(A.s[2], j); }
     // key of the slot before the move: the stored position gives exactly the
     // key the last sort used (recomputed instead of read)
     m.kold = 0;
@@ -639,13 +639,16 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
         vals[3] = ep;
         if (mp.count_moved) moved = kout != m.kold;
     }
-    st_stream(&B.p[0][i], m.x); st_stream(&B.p[1][i], m.y); st_stream(&B.p[2][i], m.z);
-    st_stream(&B.cp[0][i], m.cx); st_stream(&B.cp[1][i], m.cy); st_stream(&B.cp[2][i], m.cz);
-    st_stream(&B.v[0][i], m.vx); st_stream(&B.v[1][i], m.vy); st_stream(&B.v[2][i], m.vz);
-    st_stream(&B.cv[0][i], m.cvx); st_stream(&B.cv[1][i], m.cvy); st_stream(&B.cv[2][i], m.cvz);
-    if (mp.spin) { st_stream(&B.s[0][i], m.sx); st_stream(&B.s[1][i], m.sy); st_stream(&B.s[2][i], m.sz); }
-    st_stream(&B.id[i], m.id);
-    st_stream(&key_out[i], kout);
+    const uint32_t u = (uint32_t)i;
+    st_stream_at(B.p[0], u, m.x); st_stream_at(B.p[1], u, m.y); st_stream_at(B.p[2], u, m.z);
+    st_stream_at(B.cp[0], u, m.cx); st_stream_at(B.cp[1], u, m.cy); st_stream_at(B.cp[2], u, m.cz);
+    st_stream_at(B.v[0], u, m.vx); st_stream_at(B.v[1], u, m.vy); st_stream_at(B.v[2], u, m.vz);
+    st_stream_at(B.cv[0], u, m.cvx); st_stream_at(B.cv[1], u, m.cvy); st_stream_at(B.cv[2], u, m.cvz);
+    if (mp.spin) {
+        st_stream_at(B.s[0], u, m.sx); st_stream_at(B.s[1], u, m.sy); st_stream_at(B.s[2], u, m.sz);
+    }
+    st_stream_at(B.id, u, m.id);
+    st_stream_at(key_out, u, kout);
 }
 
 // A block of BS threads moves APT * BS consecutive slots, APT per thread: all
@@ -666,22 +669,15 @@ __global__ __launch_bounds__(BS) void k_move_fused(
     load_table(ts, trap_start, T + 1);
     const int n = *n_dev;
     const long long step = *step_dev;
-    const int b0 = blockIdx.x * (BS * APT);
-    // traps covered by this block: usually one, so no per-thread search
-    const int t_first = b0 < n ? last_le(ts, T, b0) : 0;
-    const int t_last = b0 < n ? last_le(ts, T, min(n, b0 + BS * APT) - 1) : 0;
+    // logical block: consecutive on one XCD, so an out-of-place atom's cache
+    // lines are mostly in the L2 its neighbours filled
+    const int blk = (int)blockIdx.x;
+    const int b0 = blk * (BS * APT);
+    // traps covered by this block (the same in every lane): usually one
+    const int t_first = __builtin_amdgcn_readfirstlane(b0 < n ? last_le(ts, T, b0) : 0);
+    const int t_last =
+        __builtin_amdgcn_readfirstlane(b0 < n ? last_le(ts, T, min(n, b0 + BS * APT) - 1) : 0);
     MoveAtom at[APT];
-#pragma unroll
-    for (int k = 0; k < APT; ++k) {
-        const int i = b0 + k * BS + (int)threadIdx.x;
-        at[k].t = -1;
-        if (i < n) {
-            int t = t_first;
-            while (t < t_last && ts[t + 1] <= i) ++t;
-            at[k].t = t;
-            move_load(mp, A, traps[t], perm ? perm[i] : (uint32_t)i, at[k]);
-        }
-    }
     // per atom: sum v_x^2, v_y^2, v_z^2 and U of survivors (kinetic energy and
     // the alive count follow from them and the trap ranges), and the energy of
     // evaporated atoms
@@ -692,10 +688,42 @@ __global__ __launch_bounds__(BS) void k_move_fused(
         vals[k][0] = vals[k][1] = vals[k][2] = vals[k][3] = 0.0;
         lost_e[k] = 0.0;
         lost[k] = moved[k] = false;
-        const int i = b0 + k * BS + (int)threadIdx.x;
-        if (at[k].t >= 0)
-            move_update(mp, traps[at[k].t], at[k], step, i, B, key_out, evap_pos, evap_vel, vals[k],
-                        lost_e[k], lost[k], moved[k]);
+    }
+    if (t_first == t_last) {
+        // one trap: its parameters are wave-uniform (scalar loads and branches)
+        const TrapDev &tr = traps[t_first];
+#pragma unroll
+        for (int k = 0; k < APT; ++k) {
+            const int i = b0 + k * BS + (int)threadIdx.x;
+            at[k].t = i < n ? t_first : -1;
+            if (i < n) move_load(mp, A, tr, perm ? ld_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < APT; ++k) {
+            const int i = b0 + k * BS + (int)threadIdx.x;
+            if (at[k].t >= 0)
+                move_update(mp, tr, at[k], step, i, B, key_out, evap_pos, evap_vel, vals[k],
+                            lost_e[k], lost[k], moved[k]);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < APT; ++k) {
+            const int i = b0 + k * BS + (int)threadIdx.x;
+            at[k].t = -1;
+            if (i < n) {
+                int t = t_first;
+                while (t < t_last && ts[t + 1] <= i) ++t;
+                at[k].t = t;
+                move_load(mp, A, traps[t], perm ? ld_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < APT; ++k) {
+            const int i = b0 + k * BS + (int)threadIdx.x;
+            if (at[k].t >= 0)
+                move_update(mp, traps[at[k].t], at[k], step, i, B, key_out, evap_pos, evap_vel,
+                            vals[k], lost_e[k], lost[k], moved[k]);
+        }
     }
 
     // Per-trap block sums.  Atoms are trap-major, so a block covers traps
@@ -729,8 +757,8 @@ __global__ __launch_bounds__(BS) void k_move_fused(
     if (mp.count_moved && threadIdx.x == 0) {
         uint32_t nm = 0, nd = 0;
         for (int w = 0; w < BS / kWave; ++w) { nm += wcnt[w][0]; nd += wcnt[w][1]; }
-        mp.blk_moved[blockIdx.x] = nm;
-        mp.blk_dead[blockIdx.x] = nd;
+        mp.blk_moved[blk] = nm;
+        mp.blk_dead[blk] = nd;
     }
     if (b0 < n) {
         const int nseg = t_last - t_first + 1;
@@ -753,8 +781,8 @@ __global__ __launch_bounds__(BS) void k_move_fused(
                 const double r[kAccK] = { 0.0, v[3], v[0], v[1], v[2], e[0], e[1], 0.0 };
                 if (sg == 0) {
 #pragma unroll
-                    for (int q = 0; q < kAccK; ++q) rec_sum[(size_t)q * rec_stride + blockIdx.x] = r[q];
-                    rec_trap[blockIdx.x] = t_first;
+                    for (int q = 0; q < kAccK; ++q) rec_sum[(size_t)q * rec_stride + blk] = r[q];
+                    rec_trap[blk] = t_first;
                 } else {
                     for (int q = 0; q < kAccK; ++q)
                         if (r[q] != 0.0) atomicAdd(&acc_over[tt * kAccK + q], r[q]);

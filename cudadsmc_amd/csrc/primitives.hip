@@ -251,6 +251,28 @@ __global__ __launch_bounds__(kTilesBlock) void k_ins_tiles(const uint32_t *__res
     const int tiles = n > 0 ? (n - 1) / kStayTile + 1 : 1;
     const int t0 = blockIdx.x * kTilesBlock;
     if (t0 >= tiles) return;   // whole block leaves together
+    // this thread's tile: per_tile (2 .. 32) entries, loaded before the prefix
+    const int t = t0 + (int)threadIdx.x;
+    uint32_t sm = 0, sd = 0;
+    if (t < tiles) {
+        if (per_tile >= 4) {
+            const uint4 *vm = (const uint4 *)(blk_moved + (size_t)t * per_tile);
+            const uint4 *vd = (const uint4 *)(blk_dead + (size_t)t * per_tile);
+            uint4 a[8], d[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                a[q] = q < per_tile / 4 ? vm[q] : make_uint4(0, 0, 0, 0);
+                d[q] = q < per_tile / 4 ? vd[q] : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                sm += a[q].x + a[q].y + a[q].z + a[q].w;
+                sd += d[q].x + d[q].y + d[q].z + d[q].w;
+            }
+        } else {
+            for (int b = t * per_tile; b < (t + 1) * per_tile; ++b) { sm += blk_moved[b]; sd += blk_dead[b]; }
+        }
+    }
     // earlier tiles: their move blocks [0, t0 * per_tile), a multiple of 1024
     // entries read as 16-byte vectors, four per thread in flight
     uint32_t bm = 0, bd = 0;
@@ -277,12 +299,6 @@ __global__ __launch_bounds__(kTilesBlock) void k_ins_tiles(const uint32_t *__res
     bm = tot;
     (void)block_exclusive_scan<kTilesBlock>(bd, tmp, &tot);
     bd = tot;
-    const int t = t0 + (int)threadIdx.x;
-    uint32_t sm = 0, sd = 0;
-    if (t < tiles) {
-#pragma unroll 8
-        for (int b = t * per_tile; b < (t + 1) * per_tile; ++b) { sm += blk_moved[b]; sd += blk_dead[b]; }
-    }
     uint32_t tm, td;
     const uint32_t em = block_exclusive_scan<kTilesBlock>(sm, tmp, &tm);
     const uint32_t ed = block_exclusive_scan<kTilesBlock>(sd, tmp, &td);
@@ -314,14 +330,14 @@ __device__ __forceinline__ uint32_t halfwave_exclusive_scan(uint32_t v)
 }
 
 // One thread per 64-slot mask word (a tile's 32 words are half a wave):
-// writes the word's movers (key, slot) in slot order to mov_k / mov_v and its
+// writes the slots of the word's movers, in slot order, to mov_v and its
 // removed atoms to the end of the output.  Also clears the bucket counts.
 __global__ __launch_bounds__(256) void k_ins_collect(
-    const uint32_t *__restrict__ key_new, const uint64_t *__restrict__ mask_m,
-    const uint64_t *__restrict__ mask_d, const int *__restrict__ n_dev, uint32_t dead_key,
-    const uint32_t *__restrict__ pre_m, const uint32_t *__restrict__ pre_d,
-    const int *__restrict__ counts, uint32_t *__restrict__ mov_k, uint32_t *__restrict__ mov_v,
-    uint32_t *__restrict__ out_k, uint32_t *__restrict__ out_v, uint32_t *__restrict__ bcnt)
+    const uint64_t *__restrict__ mask_m, const uint64_t *__restrict__ mask_d,
+    const int *__restrict__ n_dev, uint32_t dead_key, const uint32_t *__restrict__ pre_m,
+    const uint32_t *__restrict__ pre_d, const int *__restrict__ counts,
+    uint32_t *__restrict__ mov_v, uint32_t *__restrict__ out_k, uint32_t *__restrict__ out_v,
+    uint32_t *__restrict__ bcnt)
 {
     const int n = *n_dev;
     const int g = blockIdx.x * 256 + (int)threadIdx.x;
@@ -329,24 +345,21 @@ __global__ __launch_bounds__(256) void k_ins_collect(
     const int words = (n + 63) >> 6;
     if ((int)blockIdx.x * 256 >= words) return;   // whole block leaves together
     const bool valid = g < words;
-    const uint64_t mm = valid ? mask_m[g] : 0ull, md = valid ? mask_d[g] : 0ull;
-    const uint32_t em = halfwave_exclusive_scan((uint32_t)__popcll(mm));
-    const uint32_t ed = halfwave_exclusive_scan((uint32_t)__popcll(md));
-    if (!valid) return;
     const int tile = g >> 5;
-    uint32_t o = pre_m[tile] + em;
-    for (uint64_t b = mm; b; b &= b - 1) {
-        const uint32_t slot = (uint32_t)g * 64u + (uint32_t)(__ffsll((unsigned long long)b) - 1);
-        mov_k[o] = key_new[slot];
-        mov_v[o] = slot;
-        ++o;
+    uint64_t mm = 0ull, md = 0ull;
+    uint32_t om = 0u, od = 0u;
+    if (valid) {
+        mm = mask_m[g]; md = mask_d[g];
+        om = pre_m[tile];
+        od = (uint32_t)(n - counts[2]) + pre_d[tile];
     }
-    o = (uint32_t)(n - counts[2]) + pre_d[tile] + ed;
+    om += halfwave_exclusive_scan((uint32_t)__popcll(mm));
+    od += halfwave_exclusive_scan((uint32_t)__popcll(md));
+    for (uint64_t b = mm; b; b &= b - 1)
+        mov_v[om++] = (uint32_t)g * 64u + (uint32_t)(__ffsll((unsigned long long)b) - 1);
     for (uint64_t b = md; b; b &= b - 1) {
-        const uint32_t slot = (uint32_t)g * 64u + (uint32_t)(__ffsll((unsigned long long)b) - 1);
-        out_k[o] = dead_key;
-        out_v[o] = slot;
-        ++o;
+        out_k[od] = dead_key;
+        out_v[od++] = (uint32_t)g * 64u + (uint32_t)(__ffsll((unsigned long long)b) - 1);
     }
 }
 
@@ -362,21 +375,22 @@ __device__ __forceinline__ bool kv_less(uint32_t ka, uint32_t va, uint32_t kb, u
 // along x lands within a few places, along y or z within about Nc or Nc^2
 // cells' worth of atoms.  Also the mover's place in q's bucket.
 __global__ __launch_bounds__(256) void k_ins_locate(
-    const uint32_t *__restrict__ key_old, const uint32_t *__restrict__ mov_k,
+    const uint32_t *__restrict__ key_new, const uint32_t *__restrict__ key_old,
     const uint32_t *__restrict__ mov_v, const int *__restrict__ n_dev,
-    const int *__restrict__ counts, uint32_t *__restrict__ mov_q, uint32_t *__restrict__ mov_r,
-    uint32_t *__restrict__ bcnt)
+    const int *__restrict__ counts, uint32_t *__restrict__ mov_k, uint32_t *__restrict__ mov_q,
+    uint32_t *__restrict__ mov_r, uint32_t *__restrict__ bcnt)
 {
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int s = *n_dev, m = counts[1];
     if (blockIdx.x * 256 >= m) return;   // whole block leaves together
     const bool valid = j < m;
-    const uint32_t k = valid ? mov_k[j] : 0u, v = valid ? mov_v[j] : 0u;
+    const uint32_t v = valid ? mov_v[j] : 0u;
+    const uint32_t k = valid ? key_new[v] : 0u, k0 = valid ? key_old[v] : 0u;
     const int p0 = (int)v;
     int lo, hi;   // the insertion point lies in [lo, hi]
     if (!valid) {
         lo = hi = 0;
-    } else if (key_old[p0] < k) {   // (key_old[p0], p0) < (k, v): key_old[p0] != k
+    } else if (k0 < k) {   // (key_old[p0], p0) < (k, v): key_old[p0] != k
         int off = 1;
         while (p0 + off < s && kv_less(key_old[p0 + off], (uint32_t)(p0 + off), k, v)) off <<= 1;
         lo = p0 + (off >> 1) + 1;
@@ -401,6 +415,7 @@ __global__ __launch_bounds__(256) void k_ins_locate(
         base = atomicAdd(&bcnt[bkt], (uint32_t)__popcll(peers));
     base = __shfl(base, leader < 0 ? 0 : leader, kWave);
     if (valid) {
+        mov_k[j] = k;
         mov_q[j] = (uint32_t)lo;
         mov_r[j] = base + (uint32_t)__popcll(peers & lanemask_lt());
     }
@@ -434,11 +449,27 @@ __global__ __launch_bounds__(256) void k_ins_place(
     __shared__ uint32_t tmp[256 / kWave + 1];
     const int m = counts[1];
     const int nbk = (*n_dev >> kStayShift) + 1;
+    // this thread's first kPU movers are loaded before the bucket scan
+    constexpr int kPU = 4;
+    const int j0 = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
+    uint32_t ck[kPU], cv[kPU], cq[kPU], cr[kPU];
+#pragma unroll
+    for (int u = 0; u < kPU; ++u) {
+        const int j = j0 + u * stride;
+        const bool in = j < m;
+        ck[u] = in ? mov_k[j] : 0u; cv[u] = in ? mov_v[j] : 0u;
+        cq[u] = in ? mov_q[j] : 0u; cr[u] = in ? mov_r[j] : 0u;
+    }
     bucket_scan(bcnt, nbk, ex, tmp);
     if (blockIdx.x == 0)   // bucket offsets for the merge
         for (int i = threadIdx.x; i < nbk; i += 256) bex[i] = ex[i];
-    if (m == 0) return;
-    for (int j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) {
+#pragma unroll
+    for (int u = 0; u < kPU; ++u) {
+        if (j0 + u * stride >= m) break;
+        const uint32_t o = ex[cq[u] >> kStayShift] + cr[u];
+        bk_k[o] = ck[u]; bk_v[o] = cv[u]; bk_q[o] = cq[u];
+    }
+    for (int j = j0 + kPU * stride; j < m; j += stride) {
         const uint32_t q = mov_q[j];
         const uint32_t o = ex[q >> kStayShift] + mov_r[j];
         bk_k[o] = mov_k[j];
@@ -539,6 +570,15 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     const int tid = threadIdx.x;
     const int g0 = t * kStayTile;
     const int ns = min(kStayTile, n - g0);   // slots of the tile (0 for the bucket at n)
+    // this thread's slots g0 + tid + 256 u: their keys are loaded first, so the
+    // loads overlap the bucket's loading and ranking
+    constexpr int kU = kStayTile / 256;
+    uint32_t kk[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const int li = tid + 256 * u;
+        kk[u] = li < ns ? key_new[g0 + li] : 0u;
+    }
     if (tid < kTileWords) {
         const int l0 = tid * 64;
         uint64_t st = 0ull;
@@ -593,8 +633,10 @@ __global__ __launch_bounds__(256) void k_ins_merge(
         }
     }
     // a stay moves up by the bucket movers inserted at or before its slot
-    for (int li = tid; li < ns; li += 256) {
-        if (!((stw[li >> 6] >> (li & 63)) & 1ull)) continue;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const int li = tid + 256 * u;
+        if (li >= ns || !((stw[li >> 6] >> (li & 63)) & 1ull)) continue;
         const uint32_t g = (uint32_t)(g0 + li);
         int lo = 0, hi = b;   // first rank whose q exceeds g
         while (lo < hi) {
@@ -602,7 +644,7 @@ __global__ __launch_bounds__(256) void k_ins_merge(
             if (rq[mid] <= g) lo = mid + 1; else hi = mid;
         }
         const uint32_t o = w0 + stays_below(li) + (uint32_t)lo;
-        out_k[o] = key_new[g];
+        out_k[o] = kk[u];
         out_v[o] = g;
     }
 }
@@ -656,10 +698,10 @@ void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const 
                 stream, blk_moved, blk_dead, kStayTile / move_block, n_dev, r.tile_cnt,
                 r.tile_dead, r.counts);
     DSMC_LAUNCH(prof, "resort_collect", 0.0, k_ins_collect, div_up(words, 256), 256, stream,
-                key_new, mask_m, mask_d, n_dev, dead_key, r.tile_cnt, r.tile_dead, r.counts,
-                r.mov_k, r.mov_v, out_k, out_v, r.bcnt);
+                mask_m, mask_d, n_dev, dead_key, r.tile_cnt, r.tile_dead, r.counts, r.mov_v,
+                out_k, out_v, r.bcnt);
     DSMC_LAUNCH(prof, "resort_locate", 0.0, k_ins_locate, div_up(r.n_cap, 256), 256, stream,
-                key_old, r.mov_k, r.mov_v, n_dev, r.counts, r.mov_q, r.mov_r, r.bcnt);
+                key_new, key_old, r.mov_v, n_dev, r.counts, r.mov_k, r.mov_q, r.mov_r, r.bcnt);
     DSMC_LAUNCH(prof, "resort_place", 0.0, k_ins_place, kPlaceBlocks, 256, stream, r.mov_k,
                 r.mov_v, r.mov_q, r.mov_r, n_dev, r.counts, r.bcnt, r.bk_k, r.bk_v, r.bk_q, r.bex);
     DSMC_LAUNCH(prof, "resort_merge", n * 12.0, k_ins_merge, div_up(r.n_cap, kStayTile) + 1, 256,
