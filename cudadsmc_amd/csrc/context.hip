@@ -372,13 +372,8 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
         x->evap_accum = x->alloc<double>(2);
     } else {
         for (int b = 0; b < 2; ++b) {
-            for (int k = 0; k < 3; ++k) {
-                x->fs[b].p[k] = x->alloc<float>(N);
-                x->fs[b].v[k] = x->alloc<float>(N);
-                x->fs[b].cp[k] = x->alloc<float>(N);
-                x->fs[b].cv[k] = x->alloc<float>(N);
-                x->fs[b].s[k] = c->spin ? x->alloc<float>(N) : nullptr;
-            }
+            x->fs[b].r = x->alloc<f4>(rec_elems(N));
+            for (int k = 0; k < 3; ++k) x->fs[b].s[k] = c->spin ? x->alloc<float>(N) : nullptr;
             x->fs[b].id = x->alloc<int>(N);
         }
         resort_scratch_bind(x->res, N, x->alloc<char>(resort_scratch_bytes(N), false));
@@ -730,12 +725,14 @@ __global__ void k_observe_fused(const TrapDev *__restrict__ traps, const int *__
         if (trap_start[mid] <= i) lo = mid; else hi = mid - 1;
     }
     const TrapDev &tr = traps[lo];
-    const double X = (double)s.p[0][i] + (double)s.cp[0][i];
-    const double Y = (double)s.p[1][i] + (double)s.cp[1][i];
-    const double Z = (double)s.p[2][i] + (double)s.cp[2][i];
-    const double VX = (double)s.v[0][i] + (double)s.cv[0][i];
-    const double VY = (double)s.v[1][i] + (double)s.cv[1][i];
-    const double VZ = (double)s.v[2][i] + (double)s.cv[2][i];
+    FState f;
+    rec_load(s.r, (uint32_t)i, f);
+    const double X = (double)f.px + (double)f.rpx;
+    const double Y = (double)f.py + (double)f.rpy;
+    const double Z = (double)f.pz + (double)f.rpz;
+    const double VX = (double)f.vx + (double)f.rvx;
+    const double VY = (double)f.vy + (double)f.rvy;
+    const double VZ = (double)f.vz + (double)f.rvz;
     double *a = acc + (size_t)lo * kAccK;
     atomicAdd(&a[kAccEk], 0.5 * mass * (VX * VX + VY * VY + VZ * VZ));
     atomicAdd(&a[kAccEp], potential_energy(tr, X, Y, Z));
@@ -811,8 +808,16 @@ static InitParams init_params(const dsmc_ctx *x)
     return ip;
 }
 
+// (plane, component) of each axis' value and remainder in the FP32 state
+// records: position x, y, z = (0,0)+(0,3), (0,1)+(1,0), (0,2)+(1,1); velocity
+// x, y, z = (1,2)+(2,1), (1,3)+(2,2), (2,0)+(2,3).
+static const int kRecPosHi[3][2] = { { 0, 0 }, { 0, 1 }, { 0, 2 } };
+static const int kRecPosLo[3][2] = { { 0, 3 }, { 1, 0 }, { 1, 1 } };
+static const int kRecVelHi[3][2] = { { 1, 2 }, { 1, 3 }, { 2, 0 } };
+static const int kRecVelLo[3][2] = { { 2, 1 }, { 2, 2 }, { 2, 3 } };
+
 // ---------------------------------------------------------------------------
-// Host-side field conversion (SoA device -> AoS host)
+// Host-side field conversion (device layout -> AoS host)
 // ---------------------------------------------------------------------------
 template <typename T_>
 static std::vector<T_> d2h(const T_ *d, size_t n)
@@ -892,14 +897,17 @@ static void copy_field(dsmc_ctx *x, int field, void *host, size_t bytes)
         const FusedPtrs &s = x->fs[x->cur];
         switch (field) {
         case DSMC_FIELD_POS: case DSMC_FIELD_VEL: case DSMC_FIELD_POS_COMP: case DSMC_FIELD_VEL_COMP: {
-            float *const *hi = (field == DSMC_FIELD_POS || field == DSMC_FIELD_POS_COMP) ? s.p : s.v;
-            float *const *lo = (field == DSMC_FIELD_POS || field == DSMC_FIELD_POS_COMP) ? s.cp : s.cv;
+            const bool pos = field == DSMC_FIELD_POS || field == DSMC_FIELD_POS_COMP;
             const bool comp = field == DSMC_FIELD_POS_COMP || field == DSMC_FIELD_VEL_COMP;
+            const std::vector<f4> rec = d2h(s.r, rec_elems(N));
             for (int c = 0; c < 3; ++c) {
-                std::vector<float> a = d2h(hi[c], N), b = d2h(lo[c], N);
+                const int *h = pos ? kRecPosHi[c] : kRecVelHi[c], *l = pos ? kRecPosLo[c] : kRecVelLo[c];
                 cols[c].resize(N);
-                for (size_t i = 0; i < N; ++i)
-                    cols[c][i] = comp ? (double)b[i] : (double)a[i] + (double)b[i];
+                for (size_t i = 0; i < N; ++i) {
+                    const f4 *q = rec.data() + rec_index((uint32_t)i);
+                    const float a = q[h[0] * kRecTile][h[1]], b = q[l[0] * kRecTile][l[1]];
+                    cols[c][i] = comp ? (double)b : (double)a + (double)b;
+                }
             }
             interleave(hd, cols, 3, N);
             return;
@@ -1062,22 +1070,23 @@ dsmc_status dsmc_set_state(dsmc_ctx *ctx, const double *pos, const double *vel, 
         plain_index(ctx);
     } else {
         const FusedPtrs &s = ctx->fs[0];
-        std::vector<float> hi(6 * N), lo(6 * N);
+        std::vector<f4> rec(rec_elems(N), f4{ 0.0f, 0.0f, 0.0f, 0.0f });
         for (int c = 0; c < 3; ++c) {
             for (int which = 0; which < 2; ++which) {
                 const double *src = which ? vel : pos;
-                float *h = hi.data() + (2 * c + which) * N, *l = lo.data() + (2 * c + which) * N;
+                const int *h = which ? kRecVelHi[c] : kRecPosHi[c];
+                const int *l = which ? kRecVelLo[c] : kRecPosLo[c];
                 for (size_t i = 0; i < N; ++i) {
+                    f4 *q = rec.data() + rec_index((uint32_t)i);
                     const double v = src[3 * i + c];
-                    h[i] = (float)v;
-                    l[i] = (float)(v - (double)h[i]);
+                    const float hv = (float)v;
+                    q[h[0] * kRecTile][h[1]] = hv;
+                    q[l[0] * kRecTile][l[1]] = (float)(v - (double)hv);
                 }
-                DSMC_HIP(hipMemcpyAsync(which ? s.v[c] : s.p[c], h, N * 4, hipMemcpyHostToDevice,
-                                        ctx->stream));
-                DSMC_HIP(hipMemcpyAsync(which ? s.cv[c] : s.cp[c], l, N * 4, hipMemcpyHostToDevice,
-                                        ctx->stream));
             }
         }
+        DSMC_HIP(hipMemcpyAsync(s.r, rec.data(), rec.size() * sizeof(f4), hipMemcpyHostToDevice,
+                                ctx->stream));
         DSMC_HIP(hipMemcpyAsync(s.id, ids.data(), N * 4, hipMemcpyHostToDevice, ctx->stream));
         fused_index(ctx);
     }

@@ -91,12 +91,14 @@ __global__ __launch_bounds__(kFBlock) void k_init_fused(const TrapDev *__restric
     const TrapDev &tr = traps[t];
     double px, py, pz, vx, vy, vz;
     init_atom(tr, ip, (uint32_t)(a - ts[t]), px, py, pz, vx, vy, vz);
-    split_df(px, s.p[0][a], s.cp[0][a]);
-    split_df(py, s.p[1][a], s.cp[1][a]);
-    split_df(pz, s.p[2][a], s.cp[2][a]);
-    split_df(vx, s.v[0][a], s.cv[0][a]);
-    split_df(vy, s.v[1][a], s.cv[1][a]);
-    split_df(vz, s.v[2][a], s.cv[2][a]);
+    FState st;
+    split_df(px, st.px, st.rpx);
+    split_df(py, st.py, st.rpy);
+    split_df(pz, st.pz, st.rpz);
+    split_df(vx, st.vx, st.rvx);
+    split_df(vy, st.vy, st.rvy);
+    split_df(vz, st.vz, st.rvz);
+    rec_store(s.r, (uint32_t)a, st);
     if (ip.spin) {
         D3 Bn;
         (void)field_dir(tr, px, py, pz, Bn);
@@ -118,9 +120,11 @@ __global__ __launch_bounds__(kFBlock) void k_index_fused(const TrapDev *__restri
     if (i >= *n_dev) return;
     const int t = last_le(ts, T, i);
     const TrapDev &tr = traps[t];
-    const double X = (double)s.p[0][i] + (double)s.cp[0][i];
-    const double Y = (double)s.p[1][i] + (double)s.cp[1][i];
-    const double Z = (double)s.p[2][i] + (double)s.cp[2][i];
+    const f4 *q = s.r + rec_index((uint32_t)i);
+    const f4 p0 = q[0], p1 = q[kRecTile];
+    const double X = (double)p0.x + (double)p0.w;
+    const double Y = (double)p0.y + (double)p1.x;
+    const double Z = (double)p0.z + (double)p1.y;
     key[i] = (uint32_t)(tr.key_base + cell_of_fast(tr, X, Y, Z));
 }
 
@@ -299,11 +303,19 @@ __device__ __forceinline__ void trial_draw(const CollideParams &cp, uint32_t str
     }
 }
 
+typedef float f2 __attribute__((ext_vector_type(2)));
+
+// (vx, vy) of slot j: upper half of plane 1; vz: first float of plane 2.
+__device__ __forceinline__ f2 *vel_xy(f4 *r, uint32_t j) { return (f2 *)(r + rec_index(j) + kRecTile) + 1; }
+__device__ __forceinline__ f4 *vel_zr(f4 *r, uint32_t j) { return r + rec_index(j) + 2 * kRecTile; }
+
 __device__ __forceinline__ float rel_speed(const FusedPtrs &s, uint32_t a, uint32_t b)
 {
-    const float dx = s.v[0][a] - s.v[0][b];
-    const float dy = s.v[1][a] - s.v[1][b];
-    const float dz = s.v[2][a] - s.v[2][b];
+    const f2 ha = *vel_xy(s.r, a), hb = *vel_xy(s.r, b);
+    const float za = ((const float *)vel_zr(s.r, a))[0], zb = ((const float *)vel_zr(s.r, b))[0];
+    const float dx = ha.x - hb.x;
+    const float dy = ha.y - hb.y;
+    const float dz = za - zb;
     return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
@@ -344,24 +356,33 @@ __device__ __forceinline__ void scatter_pair(const CollideParams &cp, uint32_t s
                                              uint32_t step, int tr, const FusedPtrs &s, uint32_t a,
                                              uint32_t b)
 {
-    const double vax = (double)s.v[0][a] + (double)s.cv[0][a];
-    const double vay = (double)s.v[1][a] + (double)s.cv[1][a];
-    const double vaz = (double)s.v[2][a] + (double)s.cv[2][a];
-    const double vbx = (double)s.v[0][b] + (double)s.cv[0][b];
-    const double vby = (double)s.v[1][b] + (double)s.cv[1][b];
-    const double vbz = (double)s.v[2][b] + (double)s.cv[2][b];
+    f2 *const ha = vel_xy(s.r, a), *const hb = vel_xy(s.r, b);
+    f4 *const za = vel_zr(s.r, a), *const zb = vel_zr(s.r, b);
+    const f2 va = *ha, vb = *hb;
+    const f4 wa = *za, wb = *zb;   // (vz, rvx, rvy, rvz)
+    const double vax = (double)va.x + (double)wa.y;
+    const double vay = (double)va.y + (double)wa.z;
+    const double vaz = (double)wa.x + (double)wa.w;
+    const double vbx = (double)vb.x + (double)wb.y;
+    const double vby = (double)vb.y + (double)wb.z;
+    const double vbz = (double)wb.x + (double)wb.w;
     const double ddx = vax - vbx, ddy = vay - vby, ddz = vaz - vbz;
     const double magVrel = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
     const double cx = 0.5 * (vax + vbx), cy = 0.5 * (vay + vby), cz = 0.5 * (vaz + vbz);
     double nx, ny, nz;
     sphere_point_f(cp.seed, stream, cell, step, (uint32_t)tr, nx, ny, nz);
     const double wx = magVrel * nx, wy = magVrel * ny, wz = magVrel * nz;
-    split_df(cx - 0.5 * wx, s.v[0][a], s.cv[0][a]);
-    split_df(cy - 0.5 * wy, s.v[1][a], s.cv[1][a]);
-    split_df(cz - 0.5 * wz, s.v[2][a], s.cv[2][a]);
-    split_df(cx + 0.5 * wx, s.v[0][b], s.cv[0][b]);
-    split_df(cy + 0.5 * wy, s.v[1][b], s.cv[1][b]);
-    split_df(cz + 0.5 * wz, s.v[2][b], s.cv[2][b]);
+    FState na, nb;
+    split_df(cx - 0.5 * wx, na.vx, na.rvx);
+    split_df(cy - 0.5 * wy, na.vy, na.rvy);
+    split_df(cz - 0.5 * wz, na.vz, na.rvz);
+    split_df(cx + 0.5 * wx, nb.vx, nb.rvx);
+    split_df(cy + 0.5 * wy, nb.vy, nb.rvy);
+    split_df(cz + 0.5 * wz, nb.vz, nb.rvz);
+    *ha = f2{ na.vx, na.vy };
+    *za = f4{ na.vz, na.rvx, na.rvy, na.rvz };
+    *hb = f2{ nb.vx, nb.vy };
+    *zb = f4{ nb.vz, nb.rvx, nb.rvy, nb.rvz };
 }
 
 // The trials of one cell in sequence, by one lane (collisions.cu:397-441).
@@ -525,7 +546,8 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
 // ---------------------------------------------------------------------------
 // One atom of the fused move, in registers between its load and its update.
 struct MoveAtom {
-    float x, y, z, cx, cy, cz, vx, vy, vz, cvx, cvy, cvz, sx, sy, sz;
+    FState f;
+    float sx, sy, sz;
     int id;
     int t;          // trap
     uint32_t kold;  // key before the move (when counting)
@@ -534,10 +556,7 @@ struct MoveAtom {
 __device__ __forceinline__ void move_load(const MoveParams &mp, const FusedPtrs &A,
                                           const TrapDev &tr, uint32_t j, MoveAtom &m)
 {
-    m.x = ld_at(A.p[0], j); m.y = ld_at(A.p[1], j); m.z = ld_at(A.p[2], j);
-    m.cx = ld_at(A.cp[0], j); m.cy = ld_at(A.cp[1], j); m.cz = ld_at(A.cp[2], j);
-    m.vx = ld_at(A.v[0], j); m.vy = ld_at(A.v[1], j); m.vz = ld_at(A.v[2], j);
-    m.cvx = ld_at(A.cv[0], j); m.cvy = ld_at(A.cv[1], j); m.cvz = ld_at(A.cv[2], j);
+    rec_load(A.r, j, m.f);
     m.id = ld_at(A.id, j);
     m.sx = m.sy = m.sz = 0.0f;
     if (mp.spin) { m.sx = ld_at(A.s[0], j); m.sy = ld_at(A.s[1], j); m.sz = ld_at(A.s[2], j); }
@@ -545,9 +564,9 @@ __device__ __forceinline__ void move_load(const MoveParams &mp, const FusedPtrs 
     // key the last sort used (recomputed instead of read)
     m.kold = 0;
     if (mp.count_moved)
-        m.kold = (uint32_t)(tr.key_base + cell_of_fast(tr, (double)m.x + (double)m.cx,
-                                                       (double)m.y + (double)m.cy,
-                                                       (double)m.z + (double)m.cz));
+        m.kold = (uint32_t)(tr.key_base + cell_of_fast(tr, (double)m.f.px + (double)m.f.rpx,
+                                                       (double)m.f.py + (double)m.f.rpy,
+                                                       (double)m.f.pz + (double)m.f.rpz));
 }
 
 // Spin substeps, Kahan velocity Verlet, wrap, evaporation test, next key;
@@ -562,10 +581,10 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
 {
     const float dt = mp.f_dt;
     const float hdt = 0.5f * dt;
-    F3 a0 = accel_f(tr, m.x, m.y, m.z);
-    kahan_add(m.vx, m.cvx, a0.x * hdt);
-    kahan_add(m.vy, m.cvy, a0.y * hdt);
-    kahan_add(m.vz, m.cvz, a0.z * hdt);
+    F3 a0 = accel_f(tr, m.f.px, m.f.py, m.f.pz);
+    kahan_add(m.f.vx, m.f.rvx, a0.x * hdt);
+    kahan_add(m.f.vy, m.f.rvy, a0.y * hdt);
+    kahan_add(m.f.vz, m.f.rvz, a0.z * hdt);
 
     if (mp.spin) {
         const double dts = mp.dt / mp.substeps;
@@ -575,7 +594,8 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
         for (int k = 0; k < mp.substeps; ++k) {
             const float tau = mp.interp ? ((float)k + 0.5f) * fdts : 0.0f;
             F3 Bn;
-            const float Bm = field_dir_f(tr, m.x + m.vx * tau, m.y + m.vy * tau, m.z + m.vz * tau, Bn);
+            const float Bm = field_dir_f(tr, m.f.px + m.f.vx * tau, m.f.py + m.f.vy * tau,
+                                         m.f.pz + m.f.vz * tau, Bn);
             float sn, cs;
             sincosf(w * Bm, &sn, &cs);
             const float nd = Bn.x * sx + Bn.y * sy + Bn.z * sz;
@@ -591,31 +611,31 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
         m.sx = (float)((double)sx / L); m.sy = (float)((double)sy / L); m.sz = (float)((double)sz / L);
     }
 
-    kahan_add(m.x, m.cx, m.vx * dt);
-    kahan_add(m.y, m.cy, m.vy * dt);
-    kahan_add(m.z, m.cz, m.vz * dt);
+    kahan_add(m.f.px, m.f.rpx, m.f.vx * dt);
+    kahan_add(m.f.py, m.f.rpy, m.f.vy * dt);
+    kahan_add(m.f.pz, m.f.rpz, m.f.vz * dt);
     if (tr.periodic) {  // wrap the FP64 value, then split it again
-        double X = (double)m.x + (double)m.cx, Y = (double)m.y + (double)m.cy,
-               Z = (double)m.z + (double)m.cz;
-        if (X >= tr.hw[0] || X < -tr.hw[0]) { X = wrap_periodic(X, tr.hw[0]); split_df(X, m.x, m.cx); }
-        if (Y >= tr.hw[1] || Y < -tr.hw[1]) { Y = wrap_periodic(Y, tr.hw[1]); split_df(Y, m.y, m.cy); }
-        if (Z >= tr.hw[2] || Z < -tr.hw[2]) { Z = wrap_periodic(Z, tr.hw[2]); split_df(Z, m.z, m.cz); }
+        double X = (double)m.f.px + (double)m.f.rpx, Y = (double)m.f.py + (double)m.f.rpy,
+               Z = (double)m.f.pz + (double)m.f.rpz;
+        if (X >= tr.hw[0] || X < -tr.hw[0]) { X = wrap_periodic(X, tr.hw[0]); split_df(X, m.f.px, m.f.rpx); }
+        if (Y >= tr.hw[1] || Y < -tr.hw[1]) { Y = wrap_periodic(Y, tr.hw[1]); split_df(Y, m.f.py, m.f.rpy); }
+        if (Z >= tr.hw[2] || Z < -tr.hw[2]) { Z = wrap_periodic(Z, tr.hw[2]); split_df(Z, m.f.pz, m.f.rpz); }
     }
-    F3 a1 = accel_f(tr, m.x, m.y, m.z);
-    kahan_add(m.vx, m.cvx, a1.x * hdt);
-    kahan_add(m.vy, m.cvy, a1.y * hdt);
-    kahan_add(m.vz, m.cvz, a1.z * hdt);
+    F3 a1 = accel_f(tr, m.f.px, m.f.py, m.f.pz);
+    kahan_add(m.f.vx, m.f.rvx, a1.x * hdt);
+    kahan_add(m.f.vy, m.f.rvy, a1.y * hdt);
+    kahan_add(m.f.vz, m.f.rvz, a1.z * hdt);
 
-    const double X = (double)m.x + (double)m.cx, Y = (double)m.y + (double)m.cy,
-                 Z = (double)m.z + (double)m.cz;
-    const double VX = (double)m.vx + (double)m.cvx, VY = (double)m.vy + (double)m.cvy,
-                 VZ = (double)m.vz + (double)m.cvz;
+    const double X = (double)m.f.px + (double)m.f.rpx, Y = (double)m.f.py + (double)m.f.rpy,
+                 Z = (double)m.f.pz + (double)m.f.rpz;
+    const double VX = (double)m.f.vx + (double)m.f.rvx, VY = (double)m.f.vy + (double)m.f.rvy,
+                 VZ = (double)m.f.vz + (double)m.f.rvz;
     const double ek = 0.5 * mp.mass * (VX * VX + VY * VY + VZ * VZ);
     const double ep = potential_energy(tr, X, Y, Z);
     lost = false;
     if (mp.spin_loss) {
         F3 Bn;
-        (void)field_dir_f(tr, m.x, m.y, m.z, Bn);
+        (void)field_dir_f(tr, m.f.px, m.f.py, m.f.pz, Bn);
         lost = Bn.x * m.sx + Bn.y * m.sy + Bn.z * m.sz < 0.0f;
     }
     if (mp.energy_cut && tr.kT_cut > 0.0) {
@@ -640,10 +660,7 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
         if (mp.count_moved) moved = kout != m.kold;
     }
     const uint32_t u = (uint32_t)i;
-    st_stream_at(B.p[0], u, m.x); st_stream_at(B.p[1], u, m.y); st_stream_at(B.p[2], u, m.z);
-    st_stream_at(B.cp[0], u, m.cx); st_stream_at(B.cp[1], u, m.cy); st_stream_at(B.cp[2], u, m.cz);
-    st_stream_at(B.v[0], u, m.vx); st_stream_at(B.v[1], u, m.vy); st_stream_at(B.v[2], u, m.vz);
-    st_stream_at(B.cv[0], u, m.cvx); st_stream_at(B.cv[1], u, m.cvy); st_stream_at(B.cv[2], u, m.cvz);
+    rec_store_stream(B.r, u, m.f);
     if (mp.spin) {
         st_stream_at(B.s[0], u, m.sx); st_stream_at(B.s[1], u, m.sy); st_stream_at(B.s[2], u, m.sz);
     }
@@ -913,12 +930,13 @@ __global__ __launch_bounds__(kFBlock) void k_gather_fused(FusedPtrs dst, FusedPt
     const int i = blockIdx.x * kFBlock + threadIdx.x;
     if (i >= *n_dev) return;
     const uint32_t j = map[i];
+    const f4 *q = src.r + rec_index(j);
+    f4 *d = dst.r + rec_index((uint32_t)i);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        dst.p[c][i] = src.p[c][j]; dst.v[c][i] = src.v[c][j];
-        dst.cp[c][i] = src.cp[c][j]; dst.cv[c][i] = src.cv[c][j];
-        if (spin) dst.s[c][i] = src.s[c][j];
-    }
+    for (int c = 0; c < kRecPlanes; ++c) d[c * kRecTile] = q[c * kRecTile];
+    if (spin)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dst.s[c][i] = src.s[c][j];
     dst.id[i] = src.id[j];
 }
 

@@ -10,17 +10,62 @@
 
 namespace dsmc {
 
-// Device pointers of one FP32 state buffer (structure of arrays).  The value
-// of each coordinate is p + cp (and v + cv): the float `p` plus the Kahan
-// remainder `cp` that the previous updates could not represent.
+// FP32 state of a slot: each coordinate is a float plus the Kahan remainder
+// that the previous updates could not represent (position p + rp, velocity
+// v + rv).
+struct FState {
+    float px, py, pz, rpx, rpy, rpz;
+    float vx, vy, vz, rvx, rvy, rvz;
+};
+
+// The twelve floats of every slot are stored in tiles of 64 slots; a tile is
+// three planes of 64 four-float records,
+//   plane 0: (px, py, pz, rpx)   plane 1: (rpy, rpz, vx, vy)
+//   plane 2: (vz, rvx, rvy, rvz)
+// so a slot's state is three 16-byte accesses, a wave's access to a plane is
+// 1 KiB contiguous, and the velocity with its remainder (what a collision
+// reads and writes) is the upper half of plane 1 and plane 2.
+typedef float f4 __attribute__((ext_vector_type(4)));
+constexpr int kRecTile = 64;
+constexpr int kRecPlanes = 3;
+inline size_t rec_elems(size_t n) { return (n + kRecTile - 1) / kRecTile * kRecTile * kRecPlanes; }
+__host__ __device__ __forceinline__ size_t rec_index(uint32_t j)
+{
+    return (size_t)(j / kRecTile) * (kRecTile * kRecPlanes) + (j % kRecTile);
+}
+
+// Device pointers of one FP32 state buffer.
 struct FusedPtrs {
-    float *p[3];
-    float *v[3];
-    float *cp[3];
-    float *cv[3];
+    f4    *r;        // per-slot state records (layout above)
     float *s[3];     // Bloch spin vector (spin runs only)
     int   *id;       // atom label of each slot
 };
+
+__device__ __forceinline__ void rec_load(const f4 *r, uint32_t j, FState &a)
+{
+    const f4 *q = r + rec_index(j);
+    const f4 p0 = q[0], p1 = q[kRecTile], p2 = q[2 * kRecTile];
+    a.px = p0.x; a.py = p0.y; a.pz = p0.z; a.rpx = p0.w;
+    a.rpy = p1.x; a.rpz = p1.y; a.vx = p1.z; a.vy = p1.w;
+    a.vz = p2.x; a.rvx = p2.y; a.rvy = p2.z; a.rvz = p2.w;
+}
+
+// Non-temporal: for streaming writes of the whole state.
+__device__ __forceinline__ void rec_store_stream(f4 *r, uint32_t j, const FState &a)
+{
+    f4 *q = r + rec_index(j);
+    __builtin_nontemporal_store(f4{ a.px, a.py, a.pz, a.rpx }, q);
+    __builtin_nontemporal_store(f4{ a.rpy, a.rpz, a.vx, a.vy }, q + kRecTile);
+    __builtin_nontemporal_store(f4{ a.vz, a.rvx, a.rvy, a.rvz }, q + 2 * kRecTile);
+}
+
+__device__ __forceinline__ void rec_store(f4 *r, uint32_t j, const FState &a)
+{
+    f4 *q = r + rec_index(j);
+    q[0] = f4{ a.px, a.py, a.pz, a.rpx };
+    q[kRecTile] = f4{ a.rpy, a.rpz, a.vx, a.vy };
+    q[2 * kRecTile] = f4{ a.vz, a.rvx, a.rvy, a.rvz };
+}
 
 // Observable accumulators per trap (doubles).
 enum { kAccEk = 0, kAccEp, kAccV2x, kAccV2y, kAccV2z, kAccEevap, kAccNevap, kAccN, kAccK };
