@@ -240,9 +240,9 @@ struct CellJob {
     double   smax;    // sigma v_r,max at entry
 };
 
-__device__ __forceinline__ bool cell_setup(const TrapDev *__restrict__ traps, const int *kb, int T,
+// Cell of atoms [i, end) with composite key k in trap t (parameters tr).
+__device__ __forceinline__ bool cell_setup(const TrapDev &tr, int t, uint32_t k,
                                            const CollideParams &cp,
-                                           const uint32_t *__restrict__ key,
                                            const double *__restrict__ sigvrmax, int i, int end,
                                            CellJob &c)
 {
@@ -250,9 +250,8 @@ __device__ __forceinline__ bool cell_setup(const TrapDev *__restrict__ traps, co
     c.n = end - i;
     c.ncol = 0;
     if (c.n < 2) return false;
-    c.k = key[i];
-    c.t = last_le(kb, T + 1, (int)c.k);
-    const TrapDev &tr = traps[c.t];
+    c.k = k;
+    c.t = t;
     c.cell = c.k - (uint32_t)tr.key_base;
     if ((int)c.cell >= tr.n_cells) return false;  // overflow cell: no collisions
     c.smax = sigvrmax[c.k];
@@ -455,13 +454,32 @@ __device__ __forceinline__ int cell_trials_wave(const CollideParams &cp, const C
     return count;
 }
 
-constexpr int kCollItems = 4;  // slots per lane scanned for cell heads
+constexpr int kCollItems = 8;  // slots per lane scanned for cell heads
 constexpr int kWaveTrialsDefault = 16;
 
-// Cells with more than `wave_trials` trials are run by the whole wave after
-// the lanes have run the lighter cells one per lane.  Five waves per SIMD:
-// the kernel is bound by the latency of its dependent loads (key, sigma
-// v_r,max, permutation, velocities), and 96 registers cost a few spills only.
+// Trials of one cell (serial by one lane, or by the whole wave when heavy);
+// sigma v_r,max is stored when it grew.  Returns the accepted collisions.
+__device__ __forceinline__ int run_cell(const CollideParams &cp, const CellJob &c, uint32_t stream,
+                                        uint32_t step, const uint32_t *__restrict__ perm,
+                                        const FusedPtrs &s, double *__restrict__ sigvrmax,
+                                        bool wave)
+{
+    double smax = c.smax;
+    const int cnt = wave ? cell_trials_wave(cp, c, stream, step, perm, s, smax)
+                         : cell_trials_serial(cp, c, stream, step, perm, s, smax);
+    const bool writer = !wave || (threadIdx.x & (kWave - 1)) == 0;
+    if (writer && smax != c.smax) sigvrmax[c.k] = smax;
+    return cnt;
+}
+
+// Each wave takes kWave * kCollItems consecutive sorted slots and runs the
+// cells that start there (a cell ends where the next one starts, possibly in
+// the next chunk).  Only cells of two or more atoms are compacted for the
+// lanes (ballot + popcount), one per lane; cells with more than
+// `wave_trials` trials are then run by the whole wave.  When the chunk lies
+// in one trap, its parameters are wave-uniform (scalar loads).  Five waves
+// per SIMD: the kernel is bound by the latency of its dependent loads (key,
+// sigma v_r,max, permutation, velocities).
 __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     const TrapDev *__restrict__ traps, const int *__restrict__ key_base, int T, CollideParams cp,
     const uint32_t *__restrict__ key, const uint32_t *__restrict__ perm,
@@ -469,70 +487,112 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     unsigned long long *__restrict__ coll_accum, const long long *__restrict__ step_dev,
     int wave_trials)
 {
+    constexpr int kChunk = kWave * kCollItems;
     extern __shared__ int kb[];
-    __shared__ int heads[kFBlock / kWave][kWave * kCollItems + 1];
-    __shared__ int heavy[kFBlock / kWave][kWave * kCollItems];
+    __shared__ int heads[kFBlock / kWave][kChunk + 1];
+    __shared__ short multi[kFBlock / kWave][kChunk];   // heads index of cells with >= 2 atoms
+    __shared__ short heavy[kFBlock / kWave][kChunk];
     load_table(kb, key_base, T + 1);
     const int n = *n_dev;
     const uint32_t step = (uint32_t)*step_dev;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    const int chunk0 = (blockIdx.x * (kFBlock / kWave) + w) * (kWave * kCollItems);
-    const int chunk1 = min(n, chunk0 + kWave * kCollItems);
-    // Each wave compacts the first slots of the cells in its chunk (ballot +
-    // popcount) so that every lane then works on a different cell; a cell
-    // ends where the next one starts.
-    int nh = 0;
+    const int chunk0 = (blockIdx.x * (kFBlock / kWave) + w) * kChunk;
+    const int chunk1 = min(n, chunk0 + kChunk);
+    const uint64_t lt = lanemask_lt();
+    int nh = 0, nm = 0;
+    uint32_t klast = 0;   // key of the chunk's last cell head
     for (int it = 0; it < kCollItems; ++it) {
         const int i = chunk0 + it * kWave + lane;
-        bool head = false;
-        if (i < n) head = i == 0 || key[i - 1] != key[i];
-        const uint64_t bal = __ballot(head);
-        if (head) heads[w][nh + __popcll(bal & lanemask_lt())] = i;
-        nh += __popcll(bal);
-    }
-    if (lane == 0 && nh > 0) {
-        // end of the last cell of the chunk: gallop forward through the keys
-        const uint32_t k = key[heads[w][nh - 1]];
-        int lo = chunk1, stp = 1;
-        while (lo + stp - 1 < n && key[lo + stp - 1] == k) { lo += stp; stp <<= 1; }
-        int hi = min(n, lo + stp - 1);  // first slot with another key is in [lo, hi]
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (key[mid] == k) lo = mid + 1; else hi = mid;
+        bool head = false, mult = false;
+        uint32_t k = 0;
+        if (i < n) {
+            k = key[i];
+            head = i == 0 || key[i - 1] != k;
+            mult = head && i + 1 < n && key[i + 1] == k;
         }
-        heads[w][nh] = lo;
+        const uint64_t bh = __ballot(head), bm = __ballot(mult);
+        const int pos = nh + __popcll(bh & lt);
+        if (head) heads[w][pos] = i;
+        if (mult) multi[w][nm + __popcll(bm & lt)] = (short)pos;
+        if (bh) klast = __shfl(k, 63 - __clzll((long long)bh), kWave);
+        nh += __popcll(bh);
+        nm += __popcll(bm);
+    }
+    if (nh > 0) {
+        // end of the last cell: the first later slot with another key.  The
+        // wave probes the next 64 slots at once; a longer cell (e.g. the
+        // overflow cell of atoms outside the grid) is searched by galloping.
+        const int j = chunk1 + lane;
+        const uint64_t bd = __ballot(j < n && key[j] != klast);
+        int end = chunk1 + kWave;
+        if (bd) end = chunk1 + __ffsll((unsigned long long)bd) - 1;
+        if (chunk1 + kWave >= n) end = min(end, n);
+        if (lane == 0) {
+            if (!bd && end < n) {
+                int lo = end, stp = 1;
+                while (lo + stp - 1 < n && key[lo + stp - 1] == klast) { lo += stp; stp <<= 1; }
+                int hi = min(n, lo + stp - 1);  // the first other key is in [lo, hi]
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (key[mid] == klast) lo = mid + 1; else hi = mid;
+                }
+                end = lo;
+            }
+            heads[w][nh] = end;
+        }
+    }
+    // traps of the chunk's first and last cells: usually the same
+    int t_lo = 0, t_hi = 0;
+    if (nh > 0) {
+        t_lo = __builtin_amdgcn_readfirstlane(last_le(kb, T + 1, (int)key[chunk0]));
+        t_hi = __builtin_amdgcn_readfirstlane(last_le(kb, T + 1, (int)klast));
     }
     __syncthreads();
     int nheavy = 0;
-    for (int h0 = 0; h0 < nh; h0 += kWave) {
-        const int h = h0 + lane;
+    unsigned long long wave_cnt = 0;   // accepted collisions of trap t_lo (uniform chunks)
+    for (int m0 = 0; m0 < nm; m0 += kWave) {
+        const int mi = m0 + lane;
         CellJob c;
-        const bool ok = h < nh && cell_setup(traps, kb, T, cp, key, sigvrmax, heads[w][h],
-                                             heads[w][h + 1], c);
+        bool ok = false;
+        int h = 0;
+        if (mi < nm) {
+            h = multi[w][mi];
+            const int i = heads[w][h];
+            const uint32_t k = key[i];
+            if (t_lo == t_hi) {
+                ok = cell_setup(traps[t_lo], t_lo, k, cp, sigvrmax, i, heads[w][h + 1], c);
+            } else {
+                const int t = last_le(kb, T + 1, (int)k);
+                ok = cell_setup(traps[t], t, k, cp, sigvrmax, i, heads[w][h + 1], c);
+            }
+        }
         const bool is_heavy = ok && c.ncol > wave_trials;
         const uint64_t hb = __ballot(is_heavy);
-        if (is_heavy) heavy[w][nheavy + __popcll(hb & lanemask_lt())] = h;
+        if (is_heavy) heavy[w][nheavy + __popcll(hb & lt)] = (short)mi;
         nheavy += __popcll(hb);
         if (ok && !is_heavy) {
-            const uint32_t stream = traps[c.t].stream;
-            double smax = c.smax;
-            const int cnt = cell_trials_serial(cp, c, stream, step, perm, s, smax);
-            sigvrmax[c.k] = smax;
-            if (cnt) atomicAdd(&coll_accum[c.t], (unsigned long long)cnt);
+            const int cnt = run_cell(cp, c, traps[c.t].stream, step, perm, s, sigvrmax, false);
+            if (t_lo == t_hi) wave_cnt += (unsigned long long)cnt;
+            else if (cnt) atomicAdd(&coll_accum[c.t], (unsigned long long)cnt);
         }
     }
     __syncthreads();
     for (int q = 0; q < nheavy; ++q) {
-        const int h = heavy[w][q];
+        const int h = multi[w][heavy[w][q]];
+        const int i = heads[w][h];
+        const uint32_t k = key[i];
+        const int t = t_lo == t_hi ? t_lo : last_le(kb, T + 1, (int)k);
         CellJob c;
-        (void)cell_setup(traps, kb, T, cp, key, sigvrmax, heads[w][h], heads[w][h + 1], c);
-        const uint32_t stream = traps[c.t].stream;
-        double smax = c.smax;
-        const int cnt = cell_trials_wave(cp, c, stream, step, perm, s, smax);
+        (void)cell_setup(traps[t], t, k, cp, sigvrmax, i, heads[w][h + 1], c);
+        const int cnt = run_cell(cp, c, traps[t].stream, step, perm, s, sigvrmax, true);
         if (lane == 0) {
-            sigvrmax[c.k] = smax;
-            if (cnt) atomicAdd(&coll_accum[c.t], (unsigned long long)cnt);
+            if (t_lo == t_hi) wave_cnt += (unsigned long long)cnt;
+            else if (cnt) atomicAdd(&coll_accum[t], (unsigned long long)cnt);
         }
+    }
+    if (t_lo == t_hi) {
+        wave_cnt = wave_sum(wave_cnt);
+        if (lane == 0 && wave_cnt) atomicAdd(&coll_accum[t_lo], wave_cnt);
     }
 }
 
@@ -827,15 +887,27 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
     const int p = blockIdx.x, t = blockIdx.y;
     const long long step = *step_dev;
     const int s0 = trap_start[t], s1 = trap_start[t + 1];
+    // the per-step accumulators of the trap, loaded early (used by the block
+    // that finalizes it)
+    double over[kAccK];
+    unsigned long long coll = 0ull;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < kAccK; ++q) over[q] = acc_over[t * kAccK + q];
+        coll = coll_accum[t];
+    }
     double a[kAccK];
 #pragma unroll
     for (int q = 0; q < kAccK; ++q) a[q] = 0.0;
     if (s1 > s0) {
         const int bA = s0 / move_block, bB = (s1 - 1) / move_block;
         for (int b = bA + p * 256 + (int)threadIdx.x; b <= bB; b += P * 256) {
+            double r[kAccK];   // loaded together with the record's trap
+#pragma unroll
+            for (int q = 0; q < kAccK; ++q) r[q] = rec_sum[(size_t)q * rec_stride + b];
             if (rec_trap[b] == t) {
 #pragma unroll
-                for (int q = 0; q < kAccK; ++q) a[q] += rec_sum[(size_t)q * rec_stride + b];
+                for (int q = 0; q < kAccK; ++q) a[q] += r[q];
             }
         }
     }
@@ -868,10 +940,10 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
     if (!fin) return;   // whole block leaves together
     if (threadIdx.x == 0) {
         for (int q = 0; q < kAccK; ++q) {
-            f[q] += acc_over[t * kAccK + q];
+            f[q] += over[q];
             acc_over[t * kAccK + q] = 0.0;
         }
-        const unsigned long long c = coll_accum[t];
+        const unsigned long long c = coll;
         coll_accum[t] = 0ull;
         dsmc_observables &o = obs_log[(step % obs_cap) * T + t];
         o.ek = half_mass * (f[kAccV2x] + f[kAccV2y] + f[kAccV2z]);
