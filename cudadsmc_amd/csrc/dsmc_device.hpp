@@ -426,6 +426,40 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double *tmp)
     __syncthreads();
 }
 
+// Sums 4 doubles over a block; the totals are valid in thread 0 only.  In a
+// wave the four are reduced together: at the first two exchanges (lanes l and
+// l ^ 32, then l ^ 16) each lane keeps half of its values and sends the other
+// half, so 7 exchanges do what 24 would for one value at a time; lane l ends
+// with the wave total of quantity 2 * bit5(l) + bit4(l).  `tmp` holds
+// (BLOCK / 64) * 4 doubles.  The order of additions is fixed.
+template <int BLOCK>
+__device__ __forceinline__ void block_sum4(double (&v)[4], double *tmp)
+{
+    constexpr int W = BLOCK / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    const bool b5 = (lane & 32) != 0, b4 = (lane & 16) != 0;
+    double k0 = b5 ? v[2] : v[0], k1 = b5 ? v[3] : v[1];
+    const double s0 = b5 ? v[0] : v[2], s1 = b5 ? v[1] : v[3];
+    k0 += __shfl_xor(s0, 32, kWave);
+    k1 += __shfl_xor(s1, 32, kWave);
+    double k = b4 ? k1 : k0;
+    k += __shfl_xor(b4 ? k0 : k1, 16, kWave);
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) k += __shfl_xor(k, o, kWave);
+    if ((lane & 15) == 0) tmp[w * 4 + (b5 ? 2 : 0) + (b4 ? 1 : 0)] = k;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double t = 0.0;
+            for (int ww = 0; ww < W; ++ww) t += tmp[ww * 4 + q];
+            v[q] = t;
+        }
+    }
+    __syncthreads();
+}
+
 // Mask of the lanes whose 8-bit digit equals this lane's, among `active`.
 // Lanes of `active` whose low B bits of v equal this lane's.
 template <int B>

@@ -847,12 +847,12 @@ __global__ __launch_bounds__(BS) void k_move_fused(
                 if (at[k].t == tt)
 #pragma unroll
                     for (int q = 0; q < 4; ++q) v[q] += vals[k][q];
-            block_sum<BS, 4>(v, red);
+            block_sum4<BS>(v, red);
             if (any_lost) {
 #pragma unroll
                 for (int k = 0; k < APT; ++k)
                     if (at[k].t == tt && lost[k]) { e[0] += lost_e[k]; e[1] += 1.0; }
-                block_sum<BS, 4>(e, red);
+                block_sum4<BS>(e, red);
             }
             if (threadIdx.x == 0) {
                 const double r[kAccK] = { 0.0, v[3], v[0], v[1], v[2], e[0], e[1], 0.0 };
