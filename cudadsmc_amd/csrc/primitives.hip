@@ -547,9 +547,17 @@ __device__ int big_bucket_sort(uint32_t *k0, uint32_t *v0, uint32_t *q0, uint32_
     return src;
 }
 
-// One block per tile t: rank the tile's bucket, then write the tile's stays
-// and movers to out[] at their final positions (a contiguous window).
-// Buckets larger than the LDS table are sorted in global scratch by the block.
+// One block per tile t (2048 slots).  The tile's output is a contiguous
+// window from w0: position p of the tile (slot g0 + p) is preceded there by
+// E(p) = the stays below p plus the bucket movers inserted below p, so a stay
+// at p goes to w0 + E(p) + h(p), h(p) = movers inserted at p, and a mover
+// inserted at p to w0 + E(p) + (its rank among the movers inserted at p).  E
+// is an exclusive scan of st(p) + h(p) over the tile, h a histogram of the
+// bucket in LDS.  Buckets larger than the LDS table are sorted in global
+// scratch by the block; their movers then go to w0 + (stays below p) + (rank
+// in the bucket).
+constexpr int kMergePer = kStayTile / 256;   // slots per thread
+
 __global__ __launch_bounds__(256) void k_ins_merge(
     const uint32_t *__restrict__ key_new, const uint64_t *__restrict__ mask_m,
     const uint64_t *__restrict__ mask_d, const int *__restrict__ n_dev,
@@ -561,23 +569,24 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     uint32_t *__restrict__ alt_q, uint32_t *__restrict__ out_k, uint32_t *__restrict__ out_v)
 {
     __shared__ uint32_t lk[kBucketLds], lv[kBucketLds], lq[kBucketLds];
-    __shared__ uint32_t sk[kBucketLds], sv[kBucketLds], sq[kBucketLds];
+    __shared__ uint32_t H[kStayTile];      // histogram h, then the scan E
     __shared__ uint64_t stw[kTileWords];   // stay bits of the tile's words
     __shared__ uint32_t wst[kTileWords];   // stays of the tile before each word
+    __shared__ uint32_t tmp[256 / kWave + 1];
     const int n = *n_dev;
     const int t = blockIdx.x;
     if (t > (n >> kStayShift)) return;   // whole block leaves together
     const int tid = threadIdx.x;
     const int g0 = t * kStayTile;
     const int ns = min(kStayTile, n - g0);   // slots of the tile (0 for the bucket at n)
-    // this thread's slots g0 + tid + 256 u: their keys are loaded first, so the
-    // loads overlap the bucket's loading and ranking
-    constexpr int kU = kStayTile / 256;
-    uint32_t kk[kU];
+    const int p0 = tid * kMergePer;   // positions p0 .. p0 + 7 in the scan
+    // keys of positions tid + 256 u (coalesced), loaded first so that the
+    // loads overlap the bucket's loading
+    uint32_t kk[kMergePer];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        const int li = tid + 256 * u;
-        kk[u] = li < ns ? key_new[g0 + li] : 0u;
+    for (int u = 0; u < kMergePer; ++u) {
+        const int p = tid + 256 * u;
+        kk[u] = p < ns ? key_new[g0 + p] : 0u;
     }
     if (tid < kTileWords) {
         const int l0 = tid * 64;
@@ -590,32 +599,21 @@ __global__ __launch_bounds__(256) void k_ins_merge(
         stw[tid] = st;
         wst[tid] = halfwave_exclusive_scan((uint32_t)__popcll(st));
     }
+#pragma unroll
+    for (int u = 0; u < kMergePer; ++u) H[tid + 256 * u] = 0u;
     const uint32_t before = bex[t];   // movers in earlier buckets
     const int b = (int)bcnt[t];
     // first output slot of the tile: stays of earlier tiles + earlier movers
     const uint32_t w0 = (uint32_t)g0 - pre_m[t] - pre_d[t] + before;
     const uint32_t *bk = bk_k + before, *bv = bk_v + before, *bq = bk_q + before;
-    // stays of the tile in slots below g0 + li (li < kStayTile)
-    auto stays_below = [&](int li) -> uint32_t {
-        const uint64_t below = (li & 63) ? stw[li >> 6] & (~0ull >> (64 - (li & 63))) : 0ull;
-        return wst[li >> 6] + (uint32_t)__popcll(below);
-    };
-    const uint32_t *rq;   // bucket q values in rank order
-    if (b <= kBucketLds) {
-        for (int e = tid; e < b; e += 256) { lk[e] = bk[e]; lv[e] = bv[e]; lq[e] = bq[e]; }
-        __syncthreads();
+    const bool small = b <= kBucketLds;
+    const uint32_t *gk = nullptr, *gv = nullptr, *gq = nullptr;   // sorted big bucket
+    __syncthreads();   // H cleared, stay words in place
+    if (small) {
         for (int e = tid; e < b; e += 256) {
-            const uint32_t ke = lk[e], ve = lv[e];
-            int r = 0;
-            for (int f = 0; f < b; ++f) r += kv_less(lk[f], lv[f], ke, ve);
-            sk[r] = ke; sv[r] = ve; sq[r] = lq[e];
-        }
-        __syncthreads();
-        rq = sq;
-        for (int r = tid; r < b; r += 256) {
-            const uint32_t o = w0 + stays_below((int)sq[r] - g0) + (uint32_t)r;
-            out_k[o] = sk[r];
-            out_v[o] = sv[r];
+            const uint32_t q = bq[e];
+            lk[e] = bk[e]; lv[e] = bv[e]; lq[e] = q;
+            atomicAdd(&H[q - (uint32_t)g0], 1u);
         }
     } else {
         // copy the bucket to scratch and sort it there
@@ -624,28 +622,57 @@ __global__ __launch_bounds__(256) void k_ins_merge(
         for (int e = tid; e < b; e += 256) { ak[e] = bk[e]; av[e] = bv[e]; aq[e] = bq[e]; }
         __syncthreads();
         const int which = big_bucket_sort(ak, av, aq, ck, cv, cq, b, lk, lv, lq);
-        const uint32_t *gk = which ? ck : ak, *gv = which ? cv : av, *gq = which ? cq : aq;
-        rq = gq;
+        gk = which ? ck : ak; gv = which ? cv : av; gq = which ? cq : aq;
+        for (int r = tid; r < b; r += 256) atomicAdd(&H[gq[r] - (uint32_t)g0], 1u);
+    }
+    __syncthreads();
+    // E: exclusive scan of st(p) + h(p) over the tile, 8 consecutive
+    // positions per thread
+    const uint32_t st8 = (uint32_t)(stw[p0 >> 6] >> (p0 & 63)) & 0xFFu;
+    uint32_t e[kMergePer], run = 0;
+#pragma unroll
+    for (int u = 0; u < kMergePer; ++u) {
+        e[u] = run;
+        run += ((st8 >> u) & 1u) + H[p0 + u];
+    }
+    uint32_t total;
+    const uint32_t base = block_exclusive_scan<256>(run, tmp, &total);   // syncs: H read
+#pragma unroll
+    for (int u = 0; u < kMergePer; ++u) H[p0 + u] = base + e[u];
+    __syncthreads();   // E complete
+    // stays, at positions tid + 256 u (coalesced stores): E(p) + h(p) =
+    // E(p + 1) - st(p) = E(p + 1) - 1
+#pragma unroll
+    for (int u = 0; u < kMergePer; ++u) {
+        const int p = tid + 256 * u;
+        if (p >= ns || !((stw[p >> 6] >> (p & 63)) & 1ull)) continue;
+        const uint32_t o = w0 + (p + 1 < kStayTile ? H[p + 1] : total) - 1u;
+        out_k[o] = kk[u];
+        out_v[o] = (uint32_t)(g0 + p);
+    }
+    if (small) {
+        for (int x = tid; x < b; x += 256) {
+            const uint32_t q = lq[x], p = q - (uint32_t)g0;
+            const uint32_t ep = H[p];
+            const uint32_t stp = (uint32_t)(stw[p >> 6] >> (p & 63)) & 1u;
+            const uint32_t hp = (p + 1 < (uint32_t)kStayTile ? H[p + 1] : total) - ep - stp;
+            uint32_t r = 0;   // rank among the movers inserted at the same point
+            if (hp > 1) {
+                const uint32_t kx = lk[x], vx = lv[x];
+                for (int f = 0; f < b; ++f) r += lq[f] == q && kv_less(lk[f], lv[f], kx, vx);
+            }
+            const uint32_t o = w0 + ep + r;
+            out_k[o] = lk[x];
+            out_v[o] = lv[x];
+        }
+    } else {
         for (int r = tid; r < b; r += 256) {
-            const uint32_t o = w0 + stays_below((int)gq[r] - g0) + (uint32_t)r;
+            const int p = (int)(gq[r] - (uint32_t)g0);
+            const uint64_t below = (p & 63) ? stw[p >> 6] & (~0ull >> (64 - (p & 63))) : 0ull;
+            const uint32_t o = w0 + wst[p >> 6] + (uint32_t)__popcll(below) + (uint32_t)r;
             out_k[o] = gk[r];
             out_v[o] = gv[r];
         }
-    }
-    // a stay moves up by the bucket movers inserted at or before its slot
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        const int li = tid + 256 * u;
-        if (li >= ns || !((stw[li >> 6] >> (li & 63)) & 1ull)) continue;
-        const uint32_t g = (uint32_t)(g0 + li);
-        int lo = 0, hi = b;   // first rank whose q exceeds g
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (rq[mid] <= g) lo = mid + 1; else hi = mid;
-        }
-        const uint32_t o = w0 + stays_below(li) + (uint32_t)lo;
-        out_k[o] = kk[u];
-        out_v[o] = g;
     }
 }
 
