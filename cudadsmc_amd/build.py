@@ -2,6 +2,8 @@
 
 `build_all()` compiles, in place:
   * cudadsmc_amd/libdsmc.so      -- HIP kernels + C ABI (hipcc, gfx950)
+  * cudadsmc_amd/libdsmc_h5.so   -- HDF5 output in the reference's layout (when libhdf5
+                                    with headers is installed)
   * cudadsmc_amd/bin/cudadsmc    -- C++ host driver (CLI of main.cu:36-69)
 Every HIP source is compiled with -ffp-contract=off (FP64 results reproducible
 by the CPU restatement) and without packed FP32 instructions.
@@ -20,6 +22,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libdsmc.so"
 DRIVER = PKG / "bin" / "cudadsmc"
+H5LIB = PKG / "libdsmc_h5.so"
 
 HIP_SOURCES = ["primitives.hip", "plain.hip", "fused.hip", "context.hip"]
 HIPCC_FLAGS = [
@@ -72,14 +75,45 @@ def build_driver() -> Path:
     hipcc = _hipcc()
     DRIVER.parent.mkdir(exist_ok=True)
     src = CSRC / "driver" / "dsmc_main.cpp"
+    h5 = ["-DDSMC_WITH_H5", "-ldsmc_h5"] if H5LIB.exists() and _hdf5() is not None else []
     _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(src), "-o", str(DRIVER),
-          f"-L{PKG}", "-ldsmc", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/.."])
+          f"-L{PKG}", *h5, "-ldsmc", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/.."])
     del hipcc
     return DRIVER
 
 
+HDF5_PREFIXES = [os.environ.get("HDF5_DIR"), "/opt/conda", "/usr", "/usr/lib/x86_64-linux-gnu/hdf5/serial"]
+
+
+def _hdf5() -> Path | None:
+    """Prefix holding include/hdf5.h and lib/libhdf5.so, or None."""
+    for pre in HDF5_PREFIXES:
+        if pre and (Path(pre) / "include" / "hdf5.h").exists() and \
+                (Path(pre) / "lib" / "libhdf5.so").exists():
+            return Path(pre)
+    return None
+
+
+def build_h5() -> Path | None:
+    """libdsmc_h5.so: HDF5 output in the reference's file layout (host C++).
+    Skipped (None) when no libhdf5 with headers is installed."""
+    pre = _hdf5()
+    if pre is None:
+        return None
+    src = CSRC / "io" / "dsmc_h5.cpp"
+    tmp = H5LIB.with_suffix(".so.tmp")
+    # libstdc++ is linked statically: the run path that finds libhdf5 must not
+    # also supply an older libstdc++ than the one the HIP runtime needs
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", f"-I{ROOT / 'include'}",
+          f"-I{pre / 'include'}", str(src), "-o", str(tmp), "-static-libstdc++", "-static-libgcc",
+          f"-L{PKG}", "-ldsmc", f"-L{pre / 'lib'}", "-lhdf5", "-Wl,--enable-new-dtags",
+          f"-Wl,-rpath,$ORIGIN:{pre / 'lib'}"])
+    os.replace(tmp, H5LIB)
+    return H5LIB
+
+
 def build_all(verbose: bool = True) -> None:
-    steps = [("libdsmc.so", build_library), ("driver", build_driver)]
+    steps = [("libdsmc.so", build_library), ("libdsmc_h5.so", build_h5), ("driver", build_driver)]
     for name, fn in steps:
         out = fn()
         if verbose:

@@ -8,9 +8,12 @@
 //   cudadsmc --quadrupole N Nc STEPS [fp32|fp64]
 //                                       spherical quadrupole, B' = 2.5 T/m, 20 uK,
 //                                       grid +-5 thermal radii, dt = 1e-6 s
-//   options: --out FILE  (default outputData.bin), --prints K
+//   options: --out FILE, --prints K
 //
-// Output replaces the HDF5 file of main.cu:202-577 with a flat binary file:
+// Output: with libdsmc_h5 (built where libhdf5 is installed) the HDF5 file of
+// main.cu:202-577, default outputData.h5 (main.cu:33-34), one frame of the
+// thirteen /atomData datasets per print.  A FILE not ending in .h5, or a build
+// without libhdf5, gets a flat binary file instead:
 //   "DSMCOUT1", int32 n_capacity, int32 n_cells_plus_1, then per frame
 //   double time, int32 n_alive, double pos[n_alive][3], double vel[n_alive][3],
 //   int32 atomID[n_alive], double E_kinetic, double E_potential, int64 collisions.
@@ -22,6 +25,9 @@
 #include <vector>
 
 #include "dsmc.h"
+#ifdef DSMC_WITH_H5
+#include "dsmc_h5.h"
+#endif
 
 namespace {
 
@@ -78,7 +84,11 @@ int main(int argc, const char *argv[])
     std::memset(&cfg, 0, sizeof cfg);
     dsmc_trap trap;
     std::memset(&trap, 0, sizeof trap);
+#ifdef DSMC_WITH_H5
+    std::string out = "outputData.h5";
+#else
     std::string out = "outputData.bin";
+#endif
     int prints = 100;
     long long total_steps = 0;
     double final_time = 5e-3;
@@ -159,28 +169,57 @@ int main(int argc, const char *argv[])
     if (st != DSMC_OK) die(nullptr, st, "dsmc_create");
     CHECK(ctx, dsmc_init_distribution(ctx));
 
-    FILE *f = std::fopen(out.c_str(), "wb");
-    if (!f) {
-        std::fprintf(stderr, "cannot open %s for writing\n", out.c_str());
+    const bool h5 = out.size() > 3 && out.compare(out.size() - 3, 3, ".h5") == 0;
+    FILE *f = nullptr;
+#ifdef DSMC_WITH_H5
+    dsmc_h5 *hf = nullptr;
+    auto frame = [&](double t) {
+        if (!h5) return write_frame(f, ctx, t);
+        if (dsmc_h5_write_frame(hf, ctx, t) != DSMC_OK) {
+            std::fprintf(stderr, "HDF5 write failed: %s\n", dsmc_h5_last_error());
+            std::exit(1);
+        }
+    };
+    if (h5 && dsmc_h5_create(out.c_str(), "/atomData", &hf) != DSMC_OK) {
+        std::fprintf(stderr, "%s\n", dsmc_h5_last_error());
         dsmc_destroy(ctx);
         return 1;
     }
-    const int cells1 = trap.nc[0] * trap.nc[1] * trap.nc[2] + 1;
-    std::fwrite("DSMCOUT1", 1, 8, f);
-    std::fwrite(&trap.n_atoms, 4, 1, f);
-    std::fwrite(&cells1, 4, 1, f);
+#else
+    if (h5) {
+        std::fprintf(stderr, "this build has no HDF5 output (libhdf5 was not found)\n");
+        dsmc_destroy(ctx);
+        return 1;
+    }
+    auto frame = [&](double t) { write_frame(f, ctx, t); };
+#endif
+    if (!h5) {
+        f = std::fopen(out.c_str(), "wb");
+        if (!f) {
+            std::fprintf(stderr, "cannot open %s for writing\n", out.c_str());
+            dsmc_destroy(ctx);
+            return 1;
+        }
+        const int cells1 = trap.nc[0] * trap.nc[1] * trap.nc[2] + 1;
+        std::fwrite("DSMCOUT1", 1, 8, f);
+        std::fwrite(&trap.n_atoms, 4, 1, f);
+        std::fwrite(&cells1, 4, 1, f);
+    }
     double time = 0.0;
-    write_frame(f, ctx, time);
+    frame(time);
 
     const long long per_print = total_steps / prints;
     for (int i = 0; i < prints; ++i) {
         CHECK(ctx, dsmc_step(ctx, (int)per_print));
         time += per_print * cfg.dt;
-        write_frame(f, ctx, time);
+        frame(time);
         std::printf("%%%i complete, t = %g s\n", i * 100 / prints, time);
     }
     std::printf("\n");
-    std::fclose(f);
+    if (f) std::fclose(f);
+#ifdef DSMC_WITH_H5
+    if (hf) dsmc_h5_close(hf);
+#endif
     dsmc_destroy(ctx);
     return 0;
 }
