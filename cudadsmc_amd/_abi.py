@@ -25,7 +25,7 @@ EVAP_SPIN, EVAP_ENERGY = 1, 2
 (FIELD_POS, FIELD_VEL, FIELD_ACC, FIELD_ATOM_ID, FIELD_CELL_ID, FIELD_PERM, FIELD_SPIN,
  FIELD_EVAP_POS, FIELD_EVAP_VEL, FIELD_EVAP_TAG, FIELD_CELL_START_END, FIELD_CELL_COUNT,
  FIELD_CELL_PREFIX, FIELD_SIGVRMAX, FIELD_COLLISION_COUNT, FIELD_TRAP_START, FIELD_POS_COMP,
- FIELD_VEL_COMP, FIELD_OBS) = range(19)
+ FIELD_VEL_COMP, FIELD_OBS, FIELD_SPIN_UP, FIELD_LOCAL_POPS) = range(21)
 
 
 class Trap(C.Structure):
@@ -47,7 +47,7 @@ class Config(C.Structure):
         ("lpc", C.c_int), ("use_graph", C.c_int),
         ("dt", C.c_double), ("mass", C.c_double), ("sigma", C.c_double), ("gs", C.c_double),
         ("seed", C.c_ulonglong), ("block_size", C.c_int), ("obs_capacity", C.c_int),
-        ("sort_mode", C.c_int), ("reserved", C.c_int * 5),
+        ("sort_mode", C.c_int), ("mcwf", C.c_int), ("reserved", C.c_int * 4),
     ]
 
 
@@ -88,6 +88,9 @@ PROTOTYPES = {
     "dsmc_build_cell_table": (I, [P]),
     "dsmc_collide": (I, [P]),
     "dsmc_evolve_spin": (I, [P]),
+    "dsmc_local_populations": (I, [P]),
+    "dsmc_flip_atoms": (I, [P, C.POINTER(C.c_longlong)]),
+    "dsmc_exponential_decay": (I, [P]),
     "dsmc_move_atoms": (I, [P]),
     "dsmc_calculate_temp": (I, [P, C.POINTER(C.c_double)]),
     "dsmc_evaporation_tag": (I, [P]),

@@ -105,6 +105,7 @@ class Run:
     block_size: int = 0
     obs_capacity: int = 0
     sort_mode: str = "incremental"     # or "full": radix sort of every key each step
+    mcwf: bool = False                 # FP64 form: Majorana spin flips (wavefunction.cu:75-237)
     steps: int = 1000
     name: str = "custom"
 
@@ -134,6 +135,7 @@ class Run:
         c.block_size = int(self.block_size)
         c.obs_capacity = int(self.obs_capacity)
         c.sort_mode = _abi.SORT_INCREMENTAL if self.sort_mode == "incremental" else _abi.SORT_FULL
+        c.mcwf = int(self.mcwf)
         return c
 
 

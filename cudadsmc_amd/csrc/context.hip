@@ -100,7 +100,6 @@ Profiler::~Profiler()
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
-static constexpr double kPiRef = 3.14159265;  // deviceSystemParameters.cuh:16
 
 struct dsmc_ctx {
     dsmc_config cfg{};
@@ -153,6 +152,8 @@ struct dsmc_ctx {
     uint32_t *cmap = nullptr;
     double *evap_pos_d = nullptr, *evap_vel_d = nullptr;
     double *partials = nullptr, *evap_accum = nullptr;
+    double *pops = nullptr;                  // spin-flip model: populations by label
+    unsigned long long *flips = nullptr;     // spin-flip model: flips of the last call
     // FP32 form
     FusedPtrs fs[2];
     uint32_t *cell_start = nullptr, *cell_end = nullptr;
@@ -269,6 +270,8 @@ static void validate(const dsmc_config *c, const dsmc_trap *traps, int n_traps)
         fail(DSMC_EINVAL, "bad pair_mode");
     if (c->sort_mode != DSMC_SORT_INCREMENTAL && c->sort_mode != DSMC_SORT_FULL)
         fail(DSMC_EINVAL, "bad sort_mode");
+    if (c->mcwf && (c->precision != DSMC_FP64_PLAIN || !c->spin))
+        fail(DSMC_ENOTSUP, "the spin-flip model runs in the FP64 form with spin evolution");
     const int bs = c->block_size;
     if (bs != 0 && bs != 64 && bs != 128 && bs != 256 && bs != 512 && bs != 1024)
         fail(DSMC_EINVAL, "block_size must be 0, 64, 128, 256, 512 or 1024");
@@ -370,6 +373,10 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
         x->evap_vel_d = x->alloc<double>(3 * (size_t)N);
         x->partials = x->alloc<double>((size_t)observe_plain_blocks(N) * 5 + 8);
         x->evap_accum = x->alloc<double>(2);
+        if (c->mcwf) {
+            x->pops = x->alloc<double>(2 * (size_t)N);
+            x->flips = x->alloc<unsigned long long>(1);
+        }
     } else {
         for (int b = 0; b < 2; ++b) {
             x->fs[b].r = x->alloc<f4>(rec_elems(N));
@@ -470,6 +477,35 @@ static void plain_spin(dsmc_ctx *x)
     launch_spin_plain(x->tdev[0], sp, x->ps[x->cur], x->n_dev, x->n_cap, x->stream, &x->prof);
 }
 
+// Spin-flip model pieces (FP64 form).  The flip draws use the number of
+// collision steps taken so far (this one included) as counter word 1.
+static void plain_local_pops(dsmc_ctx *x)
+{
+    launch_local_pops_plain(x->tdev[0], x->ps[x->cur], x->pops, x->n_dev, x->n_cap, x->stream,
+                            &x->prof);
+}
+
+static void plain_flip(dsmc_ctx *x)
+{
+    FlipParams fp;
+    std::memset(&fp, 0, sizeof fp);
+    fp.seed = x->cfg.seed;
+    fp.mass = x->cfg.mass;
+    fp.gap = 2.0 * x->tdev[0].mu;
+    fp.step = (uint32_t)x->coll_steps;
+    launch_flip_plain(x->tdev[0], fp, x->ps[x->cur], x->pops, x->flips, x->n_dev, x->n_cap,
+                      x->stream, &x->prof);
+}
+
+static void plain_decay(dsmc_ctx *x)
+{
+    DecayParams dp;
+    dp.dt = x->cfg.lpc * x->cfg.dt;
+    dp.sigma = x->cfg.sigma;
+    dp.mass = x->cfg.mass;
+    launch_decay_plain(x->tdev[0], dp, x->ps[x->cur], x->n_dev, x->n_cap, x->stream, &x->prof);
+}
+
 static double e_cut_now(const dsmc_ctx *x, long long step)
 {
     const TrapDev &t = x->tdev[0];
@@ -514,6 +550,7 @@ static void plain_step(dsmc_ctx *x)
         plain_sort(x);
         plain_table(x);
         plain_collide(x);
+        if (x->cfg.mcwf) plain_local_pops(x);
     }
     if (x->cfg.spin) plain_spin(x);
     launch_move_plain(x->tdev[0], x->cfg.dt, x->ps[x->cur], x->n_dev, x->n_cap, x->stream,
@@ -521,6 +558,10 @@ static void plain_step(dsmc_ctx *x)
     if (x->cfg.evaporation) {
         plain_evap_tag(x);
         plain_compact(x);
+    }
+    if (x->cfg.mcwf && (x->step + 1) % lpc == 0) {   // main.cu:501-516
+        plain_flip(x);
+        plain_decay(x);
     }
     if (x->cfg.spin && (x->step + 1) % lpc == 0)
         launch_normalise_plain(x->ps[x->cur], x->n_dev, x->n_cap, x->stream, &x->prof);
@@ -712,6 +753,9 @@ static int host_alive(dsmc_ctx *x)
     return n;
 }
 
+// Clears one 64-bit counter (a kernel, not a memset: callers may capture it).
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
 // On-demand FP32 observables of the current state into log row obs_cap.
 __global__ void k_observe_fused(const TrapDev *__restrict__ traps, const int *__restrict__ trap_start,
                                 int T, double mass, FusedPtrs s, const int *__restrict__ n_dev,
@@ -853,6 +897,8 @@ static size_t field_size(dsmc_ctx *x, int field, int n)
     case DSMC_FIELD_TRAP_START: return (size_t)(x->T + 1) * sizeof(int);
     case DSMC_FIELD_POS_COMP: case DSMC_FIELD_VEL_COMP: return x->fp32 ? N * 3 * sizeof(double) : 0;
     case DSMC_FIELD_OBS: return (size_t)x->obs_cap * x->T * sizeof(dsmc_observables);
+    case DSMC_FIELD_SPIN_UP: return N * sizeof(int);
+    case DSMC_FIELD_LOCAL_POPS: return x->pops ? (size_t)x->n_cap * 2 * sizeof(double) : 0;
     default: return 0;
     }
 }
@@ -964,6 +1010,19 @@ static void copy_field(dsmc_ctx *x, int field, void *host, size_t bytes)
     }
     case DSMC_FIELD_OBS:
         DSMC_HIP(hipMemcpy(host, x->obs_log, need, hipMemcpyDeviceToHost));
+        return;
+    case DSMC_FIELD_SPIN_UP: {
+        int *h = (int *)host;
+        if (x->fp32) {   // the FP32 form removes atoms instead of flipping them
+            for (size_t i = 0; i < N; ++i) h[i] = 1;
+            return;
+        }
+        const std::vector<uint8_t> u = d2h(x->ps[x->cur].up, N);
+        for (size_t i = 0; i < N; ++i) h[i] = u[i] ? 1 : 0;
+        return;
+    }
+    case DSMC_FIELD_LOCAL_POPS:
+        DSMC_HIP(hipMemcpy(host, x->pops, need, hipMemcpyDeviceToHost));
         return;
     default: break;
     }
@@ -1204,6 +1263,38 @@ dsmc_status dsmc_calculate_temp(dsmc_ctx *ctx, double *T)
     observe_now(ctx, o);
     const double v2 = o[0].v2[0] + o[0].v2[1] + o[0].v2[2];
     *T = o[0].n_alive > 0 ? ctx->cfg.mass / 3. / kKB * (v2 / (double)o[0].n_alive) : 0.0;
+    DSMC_API_END(ctx)
+}
+
+dsmc_status dsmc_local_populations(dsmc_ctx *ctx)
+{
+    DSMC_API_BEGIN(ctx)
+    if (!ctx->pops) fail(DSMC_ENOTSUP, "the spin-flip model is off in this context (cfg.mcwf)");
+    plain_local_pops(ctx);
+    DSMC_API_END(ctx)
+}
+
+dsmc_status dsmc_flip_atoms(dsmc_ctx *ctx, long long *n_flipped)
+{
+    DSMC_API_BEGIN(ctx)
+    if (!ctx->pops) fail(DSMC_ENOTSUP, "the spin-flip model is off in this context (cfg.mcwf)");
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, ctx->stream, ctx->flips);
+    DSMC_HIP(hipGetLastError());
+    plain_flip(ctx);
+    if (n_flipped) {
+        unsigned long long f = 0;
+        DSMC_HIP(hipMemcpyAsync(&f, ctx->flips, sizeof f, hipMemcpyDeviceToHost, ctx->stream));
+        DSMC_HIP(hipStreamSynchronize(ctx->stream));
+        *n_flipped = (long long)f;
+    }
+    DSMC_API_END(ctx)
+}
+
+dsmc_status dsmc_exponential_decay(dsmc_ctx *ctx)
+{
+    DSMC_API_BEGIN(ctx)
+    if (!ctx->pops) fail(DSMC_ENOTSUP, "the spin-flip model is off in this context (cfg.mcwf)");
+    plain_decay(ctx);
     DSMC_API_END(ctx)
 }
 

@@ -16,6 +16,7 @@ namespace dsmc {
 constexpr double kMuB  = 9.27400915e-24;
 constexpr double kKB   = 1.3806503e-23;
 constexpr double kHbar = 1.05457148e-34;
+constexpr double kPiRef = 3.14159265;   // the reference's d_pi (deviceSystemParameters.cuh:16)
 
 constexpr int kWave = 64;
 
@@ -81,7 +82,7 @@ __device__ __forceinline__ U32x4 philox(U32x4 c, uint32_t k0, uint32_t k1)
     return c;
 }
 
-enum : uint32_t { kPurposeCollide = 1u, kPurposeInit = 2u };
+enum : uint32_t { kPurposeCollide = 1u, kPurposeInit = 2u, kPurposeFlip = 3u };
 
 // Key = (seed low word, seed high word + stream); every trap has its own
 // stream, so its random numbers do not depend on which other traps share the

@@ -242,10 +242,12 @@ dsmc_status dsmc_h5_write_frame(dsmc_h5 *h, dsmc_ctx *ctx, double time)
     std::fill(iv.begin(), iv.end(), 0);
     if ((st = copy(DSMC_FIELD_ATOM_ID, iv.data(), n * sizeof(int))) != DSMC_OK) return st;
     if ((st = put("atomID", iv.data())) != DSMC_OK) return st;
-    // removed atoms leave the state, so every alive atom is spin up
+    // spin state of the alive atoms (flipped atoms are down in the spin-flip model)
+    std::fill(iv.begin(), iv.end(), 0);
+    if ((st = copy(DSMC_FIELD_SPIN_UP, iv.data(), n * sizeof(int))) != DSMC_OK) return st;
     static_assert(sizeof(hbool_t) == 1, "one byte per flag");
     std::vector<unsigned char> sp(N, 0);
-    for (size_t i = 0; i < n; ++i) sp[i] = 1;
+    for (size_t i = 0; i < n; ++i) sp[i] = iv[i] ? 1 : 0;
     return put("atomIsSpinUp", sp.data());
 }
 

@@ -217,6 +217,123 @@ __global__ __launch_bounds__(kBlock) void k_normalise_plain(PlainPtrs s,
 }
 
 // ---------------------------------------------------------------------------
+// Majorana spin-flip model (wavefunction.cu:75-237): local populations before
+// the motion substeps of a collision step, flips and decoherence after them.
+// Populations are kept per atom label (the reference indexes them by atomID).
+// ---------------------------------------------------------------------------
+// projectLocalPopulations (wavefunction.cu:156-173): populations of the
+// spinor in the spin-up and spin-down states of the local field axis.
+__device__ __forceinline__ void local_pops(const TrapDev &t, double x, double y, double z,
+                                           double ur, double ui, double dr, double di,
+                                           double &pu, double &pd)
+{
+    D3 Bn;
+    (void)field_dir(t, x, y, z, Bn);
+    const double re = ur * dr + ui * di, im = ui * dr - ur * di;
+    const double w = 0.5 - ur * ur - ui * ui;
+    pu = 0.5 + Bn.x * re - Bn.y * im - Bn.z * w;
+    pd = 0.5 - Bn.x * re + Bn.y * im + Bn.z * w;
+}
+
+__global__ __launch_bounds__(kBlock) void k_local_pops_plain(TrapDev t, PlainPtrs s,
+                                                             double *__restrict__ pops,
+                                                             const int *__restrict__ n_dev)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= *n_dev) return;
+    double pu, pd;
+    local_pops(t, s.d[0][i], s.d[1][i], s.d[2][i], s.psi[0][i], s.psi[1][i], s.psi[2][i],
+               s.psi[3][i], pu, pd);
+    const size_t id = (size_t)s.id[i];
+    pops[2 * id] = pu;
+    pops[2 * id + 1] = pd;
+}
+
+// flipAtoms (wavefunction.cu:99-154): an atom changes spin state with the
+// probability that its population moved into the other state since the
+// populations were taken, if its kinetic energy can pay the potential step
+// 2 mu |B| (gs muB |B| for the reference's mu); the speed then absorbs the
+// step along the old direction.  The uniform is Philox (label, step, 0,
+// purpose 3).
+__global__ __launch_bounds__(kBlock) void k_flip_plain(TrapDev t, FlipParams fp, PlainPtrs s,
+                                                       const double *__restrict__ pops,
+                                                       unsigned long long *__restrict__ flips,
+                                                       const int *__restrict__ n_dev)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    unsigned long long cnt = 0ull;
+    if (i < *n_dev) {
+        const double x = s.d[0][i], y = s.d[1][i], z = s.d[2][i];
+        const double vx = s.d[3][i], vy = s.d[4][i], vz = s.d[5][i];
+        const uint32_t id = (uint32_t)s.id[i];
+        const bool up = s.up[i] != 0;
+        const double opu = pops[2 * (size_t)id], opd = pops[2 * (size_t)id + 1];
+        double npu, npd;
+        local_pops(t, x, y, z, s.psi[0][i], s.psi[1][i], s.psi[2][i], s.psi[3][i], npu, npd);
+        D3 Bn;
+        const double magB = field_dir(t, x, y, z, Bn);
+        double Pflip, dEp;
+        if (up) {
+            Pflip = (npd - opd) / opu;
+            dEp = fp.gap * magB;
+        } else {
+            Pflip = (npu - opu) / opd;
+            dEp = (-1.0 * fp.gap) * magB;
+        }
+        const double Ek = 0.5 * fp.mass * (vx * vx + vy * vy + vz * vz);
+        const U32x4 o = draw(fp.seed, t.stream, id, fp.step, 0u, kPurposeFlip, 0u);
+        if (Pflip > u01(o.x, o.y) && Ek + dEp > 0.0) {
+            s.up[i] = up ? 0 : 1;
+            const double magVel = sqrt(vx * vx + vy * vy + vz * vz);
+            const double speed = sqrt(2.0 * (Ek + dEp) / fp.mass);
+            if (magVel > 0.0) {
+                s.d[3][i] = speed * (vx / magVel);
+                s.d[4][i] = speed * (vy / magVel);
+                s.d[5][i] = speed * (vz / magVel);
+            } else {   // no direction to keep: along +z
+                s.d[3][i] = 0.0; s.d[4][i] = 0.0; s.d[5][i] = speed;
+            }
+            cnt = 1ull;
+        }
+    }
+    cnt = wave_sum(cnt);
+    if ((threadIdx.x & (kWave - 1)) == 0 && cnt) atomicAdd(flips, cnt);
+}
+
+// calculateTau (wavefunction.cu:206-219) with getAcc (:221-237), the spin-up
+// acceleration: the decoherence time from the position- and velocity-limited
+// times of an atom that the field gradient separates.
+__device__ __forceinline__ double decoherence_tau(const TrapDev &t, double sigma, double mass,
+                                                  double x, double y, double z)
+{
+    const D3 a = accel(t, x, y, z, true);
+    const double relAccel = 2.0 * sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    const double tauPos = 1.163 * sqrt(sigma / relAccel);
+    const double tauVel = sqrt(2. / kPiRef) * kHbar / (mass * sigma * relAccel);
+    return pow(1 / (tauPos * tauPos * tauPos) + 1 / (tauVel * tauVel * tauVel), -1. / 3.);
+}
+
+// exponentialDecay (wavefunction.cu:175-204): psi <- P+ psi + e P- psi with
+// P+- = (1 +- B.sigma) / 2 and e = exp(-dt / tau), written out per component.
+__global__ __launch_bounds__(kBlock) void k_decay_plain(TrapDev t, DecayParams dp, PlainPtrs s,
+                                                        const int *__restrict__ n_dev)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= *n_dev) return;
+    const double x = s.d[0][i], y = s.d[1][i], z = s.d[2][i];
+    const double ur = s.psi[0][i], ui = s.psi[1][i], dr = s.psi[2][i], di = s.psi[3][i];
+    D3 Bn;
+    (void)field_dir(t, x, y, z, Bn);
+    const double tau = decoherence_tau(t, dp.sigma, dp.mass, x, y, z);
+    const double om = 1. - exp(-dp.dt / tau);
+    const double op = 1. + exp(-dp.dt / tau);
+    s.psi[0][i] = 0.5 * ((Bn.x * dr + Bn.y * di + Bn.z * ur) * om + ur * op);
+    s.psi[1][i] = 0.5 * ((Bn.x * di - Bn.y * dr + Bn.z * ui) * om + ui * op);
+    s.psi[2][i] = 0.5 * ((Bn.x * ur - Bn.y * ui - Bn.z * dr) * om + dr * op);
+    s.psi[3][i] = 0.5 * ((Bn.x * ui + Bn.y * ur - Bn.z * di) * om + di * op);
+}
+
+// ---------------------------------------------------------------------------
 // evaporationTag (evaporation.cu:68-106) plus the energy cut.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_evap_tag_plain(TrapDev t, EvapParams ep, PlainPtrs s,
@@ -280,7 +397,10 @@ __global__ __launch_bounds__(kBlock) void k_observe_plain(TrapDev t, double mass
         const double x = s.d[0][i], y = s.d[1][i], z = s.d[2][i];
         const double vx = s.d[3][i], vy = s.d[4][i], vz = s.d[5][i];
         v[0] = 0.5 * mass * (vx * vx + vy * vy + vz * vz);
-        v[1] = potential_energy(t, x, y, z);
+        // U_up = mu (|B| - B0); a spin-down atom (after a flip) has
+        // U_down = -mu (|B| + B0), so that a flip conserves E_k + U
+        const double u = potential_energy(t, x, y, z);
+        v[1] = s.up[i] ? u : -u - 2.0 * t.mu * t.B0;
         v[2] = vx * vx; v[3] = vy * vy; v[4] = vz * vz;
     }
     block_sum<kBlock, kObsK>(v, tmp);
@@ -428,6 +548,28 @@ void launch_evap_tag_plain(const TrapDev &t, const EvapParams &ep, const PlainPt
 {
     DSMC_LAUNCH(prof, "evap_tag_plain", (double)n_cap * 84.0, k_evap_tag_plain, grid_for(n_cap),
                 kBlock, st, t, ep, s, tag, evap_pos, evap_vel, evap_accum, n_dev);
+}
+
+void launch_local_pops_plain(const TrapDev &t, const PlainPtrs &s, double *pops, const int *n_dev,
+                             int n_cap, hipStream_t st, Profiler *prof)
+{
+    DSMC_LAUNCH(prof, "local_pops_plain", (double)n_cap * 76.0, k_local_pops_plain,
+                grid_for(n_cap), kBlock, st, t, s, pops, n_dev);
+}
+
+void launch_flip_plain(const TrapDev &t, const FlipParams &fp, const PlainPtrs &s,
+                       const double *pops, unsigned long long *flips, const int *n_dev, int n_cap,
+                       hipStream_t st, Profiler *prof)
+{
+    DSMC_LAUNCH(prof, "flip_plain", (double)n_cap * 101.0, k_flip_plain, grid_for(n_cap), kBlock,
+                st, t, fp, s, pops, flips, n_dev);
+}
+
+void launch_decay_plain(const TrapDev &t, const DecayParams &dp, const PlainPtrs &s,
+                        const int *n_dev, int n_cap, hipStream_t st, Profiler *prof)
+{
+    DSMC_LAUNCH(prof, "decay_plain", (double)n_cap * 88.0, k_decay_plain, grid_for(n_cap), kBlock,
+                st, t, dp, s, n_dev);
 }
 
 int observe_plain_blocks(int n_cap) { return grid_for(n_cap); }

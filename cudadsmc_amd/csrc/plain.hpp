@@ -33,6 +33,20 @@ struct SpinParams {
     int    interp;
 };
 
+struct FlipParams {
+    unsigned long long seed;
+    double mass;
+    double gap;         // potential step of a flip per |B|: 2 mu (J/T)
+    uint32_t step;      // Philox counter word 1
+    int    pad;
+};
+
+struct DecayParams {
+    double dt;          // lpc * dt
+    double sigma;
+    double mass;
+};
+
 struct EvapParams {
     double mass;
     double e_cut;       // J, <= 0: off
@@ -57,6 +71,13 @@ void launch_spin_plain(const TrapDev &t, const SpinParams &sp, const PlainPtrs &
                        const int *n_dev, int n_cap, hipStream_t st, Profiler *prof);
 void launch_normalise_plain(const PlainPtrs &s, const int *n_dev, int n_cap, hipStream_t st,
                             Profiler *prof);
+void launch_local_pops_plain(const TrapDev &t, const PlainPtrs &s, double *pops, const int *n_dev,
+                             int n_cap, hipStream_t st, Profiler *prof);
+void launch_flip_plain(const TrapDev &t, const FlipParams &fp, const PlainPtrs &s,
+                       const double *pops, unsigned long long *flips, const int *n_dev, int n_cap,
+                       hipStream_t st, Profiler *prof);
+void launch_decay_plain(const TrapDev &t, const DecayParams &dp, const PlainPtrs &s,
+                        const int *n_dev, int n_cap, hipStream_t st, Profiler *prof);
 void launch_evap_tag_plain(const TrapDev &t, const EvapParams &ep, const PlainPtrs &s, int *tag,
                            double *evap_pos, double *evap_vel, double *evap_accum,
                            const int *n_dev, int n_cap, hipStream_t st, Profiler *prof);

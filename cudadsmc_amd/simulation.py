@@ -84,6 +84,20 @@ class Simulation:
     def evolve_spin(self):
         self._check(self.lib.dsmc_evolve_spin(self.h))
 
+    def local_populations(self):
+        """getLocalPopulations (wavefunction.cu:75-97), cfg.mcwf."""
+        self._check(self.lib.dsmc_local_populations(self.h))
+
+    def flip_atoms(self) -> int:
+        """flipAtoms (wavefunction.cu:99-154); returns the atoms flipped."""
+        n = C.c_longlong()
+        self._check(self.lib.dsmc_flip_atoms(self.h, C.byref(n)))
+        return n.value
+
+    def exponential_decay(self):
+        """exponentialDecay over lpc * dt (wavefunction.cu:175-219)."""
+        self._check(self.lib.dsmc_exponential_decay(self.h))
+
     def move_atoms(self):
         self._check(self.lib.dsmc_move_atoms(self.h))
 
@@ -127,7 +141,8 @@ class Simulation:
         _abi.FIELD_CELL_COUNT: (np.int32, 1), _abi.FIELD_CELL_PREFIX: (np.int32, 1),
         _abi.FIELD_SIGVRMAX: (np.float64, 1), _abi.FIELD_COLLISION_COUNT: (np.int32, 1),
         _abi.FIELD_TRAP_START: (np.int32, 1), _abi.FIELD_POS_COMP: (np.float64, 3),
-        _abi.FIELD_VEL_COMP: (np.float64, 3),
+        _abi.FIELD_VEL_COMP: (np.float64, 3), _abi.FIELD_SPIN_UP: (np.int32, 1),
+        _abi.FIELD_LOCAL_POPS: (np.float64, 2),
     }
 
     def copy_out(self, field: int) -> np.ndarray:

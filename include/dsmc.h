@@ -96,7 +96,10 @@ typedef struct dsmc_config {
     int    block_size;      /* per-atom kernels; 0 = default                      */
     int    obs_capacity;    /* steps kept in the observables ring; 0 = default    */
     int    sort_mode;       /* DSMC_SORT_* (FP32 form)                            */
-    int    reserved[5];
+    int    mcwf;            /* FP64 form with spin: Majorana spin-flip model of   */
+                            /* wavefunction.cu:75-237 (populations, flips,        */
+                            /* decoherence) in every collision step               */
+    int    reserved[4];
 } dsmc_config;
 
 /* FP64 observables of one trap after one motion step. */
@@ -149,6 +152,8 @@ enum {
     DSMC_FIELD_POS_COMP,       /* n x 3 double: Kahan remainders (FP32 form)       */
     DSMC_FIELD_VEL_COMP,       /* n x 3 double: Kahan remainders (FP32 form)       */
     DSMC_FIELD_OBS,            /* steps x n_traps dsmc_observables (whole log)     */
+    DSMC_FIELD_SPIN_UP,        /* n int: 1 spin up, 0 spin down (atomIsSpinUp)     */
+    DSMC_FIELD_LOCAL_POPS,     /* n_capacity x 2 double, by label (FP64 form, mcwf) */
     DSMC_FIELD_COUNT_
 };
 
@@ -197,6 +202,14 @@ dsmc_status dsmc_build_cell_table(dsmc_ctx *ctx);
 dsmc_status dsmc_collide(dsmc_ctx *ctx);
 /* evolveWavefunction + normalise (main.cu:465-469, 518-521). */
 dsmc_status dsmc_evolve_spin(dsmc_ctx *ctx);
+/* Majorana spin-flip model (FP64 form, cfg.mcwf): getLocalPopulations
+ * (wavefunction.cu:75-97, main.cu:457-462) before the motion substeps of a
+ * collision step; flipAtoms (wavefunction.cu:99-154, main.cu:501-509), with
+ * *n_flipped (may be NULL) the atoms flipped, and exponentialDecay over
+ * lpc * dt (wavefunction.cu:175-219, main.cu:511-516) after them. */
+dsmc_status dsmc_local_populations(dsmc_ctx *ctx);
+dsmc_status dsmc_flip_atoms(dsmc_ctx *ctx, long long *n_flipped);
+dsmc_status dsmc_exponential_decay(dsmc_ctx *ctx);
 /* moveAtoms (moveAtoms.cu:23-49).  FP32 form: also evolves spins, computes the
  * next cell keys, applies evaporation and accumulates the FP64 observables. */
 dsmc_status dsmc_move_atoms(dsmc_ctx *ctx);
@@ -208,7 +221,9 @@ dsmc_status dsmc_evaporation_tag(dsmc_ctx *ctx);
  * tagged atoms; *n_alive (may be NULL) receives the new count. */
 dsmc_status dsmc_compact(dsmc_ctx *ctx, int *n_alive);
 /* The body of main.cu:412-523 repeated: n_steps motion steps, with a collision
- * step every lpc motion steps.  Nothing is copied to the host. */
+ * step every lpc motion steps (and, with cfg.mcwf, populations before and
+ * flips and decoherence after its motion substeps).  Nothing is copied to the
+ * host. */
 dsmc_status dsmc_step(dsmc_ctx *ctx, int n_steps);
 /* Block until all work on the context's stream is done. */
 dsmc_status dsmc_sync(dsmc_ctx *ctx);

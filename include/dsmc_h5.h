@@ -42,7 +42,7 @@ dsmc_status dsmc_h5_append(dsmc_h5 *h, const char *name, const void *data);
  * double), collisionCount, atomCount (cells + 1 int), simuatedTime (1 double),
  * atomNumber (1 int), cellID, atomID (n int), atomIsSpinUp (n bool).  n is the
  * context's capacity (the reference's numberOfAtoms at creation); rows past
- * the alive atoms are zero.  The datasets are created by the first call.
+ * the alive atoms are zero.  atomIsSpinUp is DSMC_FIELD_SPIN_UP.  The datasets are created by the first call.
  * Fields that the context's numerical form does not hold are written as
  * zeros: the spinor of an FP32 run is the Bloch vector's spinor (phase of
  * psiUp zero), per-cell counts exist in the FP64 form only. */

@@ -59,6 +59,7 @@ uint32_t orc_uint_below(uint32_t r, uint32_t n)
 /* Counter word 3 carries (purpose << 24) | draw index. */
 #define ORC_PURPOSE_COLLIDE 1u
 #define ORC_PURPOSE_INIT    2u
+#define ORC_PURPOSE_FLIP    3u
 
 /* Key = (seed low word, seed high word + stream): each trap has its own
  * Philox stream.  Counter = (a, b, c, (purpose << 24) | draw). */
@@ -420,6 +421,109 @@ void orc_normalise(orc_c *psiUp, orc_c *psiDn, const int *atomID, int n)
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Majorana spin-flip model (src/wavefunction.cu:75-237, main.cu:457-462,    */
+/* 501-516).  Populations are stored per atom label, two doubles each.       */
+/* ------------------------------------------------------------------------ */
+/* projectLocalPopulations (wavefunction.cu:156-173). */
+static void local_pops(const orc_params *p, orc_d3 r, orc_c u, orc_c d, double *pu, double *pd)
+{
+    orc_d3 Bn;
+    (void)field_mag_dir(p, r, &Bn);
+    double re = u.re * d.re + u.im * d.im, im = u.im * d.re - u.re * d.im;
+    double w = 0.5 - u.re * u.re - u.im * u.im;
+    *pu = 0.5 + Bn.x * re - Bn.y * im - Bn.z * w;
+    *pd = 0.5 - Bn.x * re + Bn.y * im + Bn.z * w;
+}
+
+/* getLocalPopulations (wavefunction.cu:75-97). */
+void orc_local_populations(const orc_params *p, const orc_d3 *pos, const orc_c *psiUp,
+                           const orc_c *psiDn, const int *atomID, int n, double *pops)
+{
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < n; ++s) {
+        int a = atomID[s];
+        local_pops(p, pos[a], psiUp[a], psiDn[a], &pops[2 * (size_t)a], &pops[2 * (size_t)a + 1]);
+    }
+}
+
+/* flipAtoms (wavefunction.cu:99-154): flip probability from the change of the
+ * populations, potential step 2 mu |B| (gs muB |B| for mu = gs muB / 2), the
+ * speed absorbs it along the old direction (along +z from rest).  Uniform:
+ * Philox counter (label, step, 0, (3 << 24) | 0). */
+long long orc_flip_atoms(const orc_params *p, const orc_d3 *pos, orc_d3 *vel, const orc_c *psiUp,
+                         const orc_c *psiDn, const double *pops, uint8_t *spinUp,
+                         const int *atomID, int n, uint32_t step)
+{
+    long long flips = 0;
+#pragma omp parallel for schedule(static) reduction(+ : flips)
+    for (int s = 0; s < n; ++s) {
+        int a = atomID[s];
+        orc_d3 r = pos[a], v = vel[a];
+        double opu = pops[2 * (size_t)a], opd = pops[2 * (size_t)a + 1], npu, npd;
+        local_pops(p, r, psiUp[a], psiDn[a], &npu, &npd);
+        orc_d3 Bn;
+        double magB = field_mag_dir(p, r, &Bn);
+        double gap = 2.0 * p->mu, Pflip, dEp;
+        int up = spinUp[a] != 0;
+        if (up) {
+            Pflip = (npd - opd) / opu;
+            dEp = gap * magB;
+        } else {
+            Pflip = (npu - opu) / opd;
+            dEp = (-1.0 * gap) * magB;
+        }
+        double Ek = 0.5 * p->mass * (v.x * v.x + v.y * v.y + v.z * v.z);
+        uint32_t o[4];
+        orc_draw(p->seed, p->stream, (uint32_t)a, step, 0u, ORC_PURPOSE_FLIP, 0u, o);
+        if (Pflip > orc_u01(o[0], o[1]) && Ek + dEp > 0.0) {
+            spinUp[a] = up ? 0 : 1;
+            double magVel = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+            double speed = sqrt(2.0 * (Ek + dEp) / p->mass);
+            if (magVel > 0.0) {
+                vel[a].x = speed * (v.x / magVel);
+                vel[a].y = speed * (v.y / magVel);
+                vel[a].z = speed * (v.z / magVel);
+            } else {
+                vel[a].x = 0.0; vel[a].y = 0.0; vel[a].z = speed;
+            }
+            flips += 1;
+        }
+    }
+    return flips;
+}
+
+/* calculateTau (wavefunction.cu:206-219) with getAcc (:221-237) = the spin-up
+ * acceleration of orc_accel. */
+double orc_decoherence_tau(const orc_params *p, orc_d3 r)
+{
+    orc_d3 g = orc_accel(p, r, 1);
+    double relAccel = 2.0 * sqrt(g.x * g.x + g.y * g.y + g.z * g.z);
+    double tauPos = 1.163 * sqrt(p->sigma / relAccel);
+    double tauVel = sqrt(2. / ORC_PI) * ORC_HBAR / (p->mass * p->sigma * relAccel);
+    return pow(1 / (tauPos * tauPos * tauPos) + 1 / (tauVel * tauVel * tauVel), -1. / 3.);
+}
+
+/* exponentialDecay (wavefunction.cu:175-204). */
+void orc_exponential_decay(const orc_params *p, const orc_d3 *pos, orc_c *psiUp, orc_c *psiDn,
+                           const int *atomID, int n, double dt)
+{
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < n; ++s) {
+        int a = atomID[s];
+        orc_d3 Bn;
+        (void)field_mag_dir(p, pos[a], &Bn);
+        double tau = orc_decoherence_tau(p, pos[a]);
+        double om = 1. - exp(-dt / tau), op = 1. + exp(-dt / tau);
+        orc_c u = psiUp[a], d = psiDn[a], nu, nd;
+        nu.re = 0.5 * ((Bn.x * d.re + Bn.y * d.im + Bn.z * u.re) * om + u.re * op);
+        nu.im = 0.5 * ((Bn.x * d.im - Bn.y * d.re + Bn.z * u.im) * om + u.im * op);
+        nd.re = 0.5 * ((Bn.x * u.re - Bn.y * u.im - Bn.z * d.re) * om + d.re * op);
+        nd.im = 0.5 * ((Bn.x * u.im + Bn.y * u.re - Bn.z * d.im) * om + d.im * op);
+        psiUp[a] = nu; psiDn[a] = nd;
+    }
+}
+
 /* Bloch-vector form of the same rotation: the SU(2) operator
  * cos(theta) - i sin(theta) Bn.sigma of wavefunction.cu:47-70 rotates
  * s = (2 Re u*d, 2 Im u*d, |u|^2 - |d|^2) by phi = 2 theta about Bn
@@ -501,6 +605,14 @@ double orc_calculate_temp(const orc_params *p, const orc_d3 *vel, const int *ato
 void orc_energy(const orc_params *p, const orc_d3 *pos, const orc_d3 *vel,
                 const int *atomID, int n, double out[3])
 {
+    orc_energy_spin(p, pos, vel, NULL, atomID, n, out);
+}
+
+/* With spin states: U_down = -mu (|B| + B0) = -U_up - 2 mu B0, so that a flip
+ * (potential step 2 mu |B|) conserves E_k + U.  spinUp NULL: all up. */
+void orc_energy_spin(const orc_params *p, const orc_d3 *pos, const orc_d3 *vel,
+                     const uint8_t *spinUp, const int *atomID, int n, double out[3])
+{
     double ek = 0.0, ep = 0.0, v2 = 0.0;
 #pragma omp parallel for schedule(static) reduction(+ : ek, ep, v2)
     for (int s = 0; s < n; ++s) {
@@ -509,7 +621,8 @@ void orc_energy(const orc_params *p, const orc_d3 *pos, const orc_d3 *vel,
         double w = v.x * v.x + v.y * v.y + v.z * v.z;
         v2 += w;
         ek += 0.5 * p->mass * w;
-        ep += orc_potential(p, pos[a]);
+        double u = orc_potential(p, pos[a]);
+        ep += (!spinUp || spinUp[a]) ? u : -u - 2.0 * p->mu * p->B0;
     }
     out[0] = ek; out[1] = ep; out[2] = v2;
 }

@@ -92,6 +92,15 @@ void orc_move_atoms(const orc_params *p, orc_d3 *pos, orc_d3 *vel, orc_d3 *acc,
 void orc_evolve_wavefunction(const orc_params *p, const orc_d3 *pos, orc_c *psiUp,
                              orc_c *psiDn, const int *atomID, int n, double dt_spin);
 void orc_normalise(orc_c *psiUp, orc_c *psiDn, const int *atomID, int n);
+/* Majorana spin-flip model (wavefunction.cu:75-237); pops: 2 per label. */
+void orc_local_populations(const orc_params *p, const orc_d3 *pos, const orc_c *psiUp,
+                           const orc_c *psiDn, const int *atomID, int n, double *pops);
+long long orc_flip_atoms(const orc_params *p, const orc_d3 *pos, orc_d3 *vel, const orc_c *psiUp,
+                         const orc_c *psiDn, const double *pops, uint8_t *spinUp,
+                         const int *atomID, int n, uint32_t step);
+double orc_decoherence_tau(const orc_params *p, orc_d3 pos);
+void orc_exponential_decay(const orc_params *p, const orc_d3 *pos, orc_c *psiUp, orc_c *psiDn,
+                           const int *atomID, int n, double dt);
 void orc_rotate_bloch(const orc_params *p, const orc_d3 *pos, orc_d3 *s,
                       const int *atomID, int n, double dt_spin);
 void orc_evaporation_tag(const orc_params *p, const orc_d3 *pos, const orc_d3 *vel,
@@ -102,6 +111,8 @@ int  orc_remove_if(int *atomID, const int *evapTag, int n);
 double orc_calculate_temp(const orc_params *p, const orc_d3 *vel, const int *atomID, int n);
 void orc_energy(const orc_params *p, const orc_d3 *pos, const orc_d3 *vel,
                 const int *atomID, int n, double out[3] /* Ek, Ep, sum v^2 */);
+void orc_energy_spin(const orc_params *p, const orc_d3 *pos, const orc_d3 *vel,
+                     const uint8_t *spinUp, const int *atomID, int n, double out[3]);
 
 /* ---- initial conditions (setUp.cu:129-252, 287-292) ------------------- */
 void orc_init_distribution(const orc_params *p, double T, const double Taniso[3],

@@ -67,6 +67,11 @@ def load():
             "orc_remove_if": (C.c_int, [P, P, C.c_int]),
             "orc_calculate_temp": (C.c_double, [P, P, P, C.c_int]),
             "orc_energy": (None, [P, P, P, P, C.c_int, P]),
+            "orc_energy_spin": (None, [P, P, P, P, P, C.c_int, P]),
+            "orc_local_populations": (None, [P, P, P, P, P, C.c_int, P]),
+            "orc_flip_atoms": (C.c_longlong, [P, P, P, P, P, P, P, P, C.c_int, C.c_uint32]),
+            "orc_decoherence_tau": (C.c_double, [P, D3]),
+            "orc_exponential_decay": (None, [P, P, P, P, P, C.c_int, C.c_double]),
             "orc_init_distribution": (None, [P, C.c_double, P, P, P, P, P, P, P, P, C.c_int]),
             "orc_sigvrmax_init": (C.c_double, [P, C.c_double]),
             "orc_run": (None, [P, P, P, P, P, P, P, C.c_int, C.c_int, C.c_uint32, P]),
@@ -235,11 +240,46 @@ def calculate_temp(p: Params, vel, atom_id):
     return load().orc_calculate_temp(C.byref(p), _p(vel), _p(atom_id), len(atom_id))
 
 
-def energy(p: Params, pos, vel, atom_id):
+def energy(p: Params, pos, vel, atom_id, spin_up=None):
+    """(E_k, E_p, sum v^2); with spin states, U_down = -U_up - 2 mu B0."""
     atom_id = np.ascontiguousarray(atom_id, np.int32)
     out = np.zeros(3)
-    load().orc_energy(C.byref(p), _p(pos), _p(vel), _p(atom_id), len(atom_id), _p(out))
+    if spin_up is None:
+        load().orc_energy(C.byref(p), _p(pos), _p(vel), _p(atom_id), len(atom_id), _p(out))
+    else:
+        up = np.ascontiguousarray(spin_up, np.uint8)
+        load().orc_energy_spin(C.byref(p), _p(pos), _p(vel), _p(up), _p(atom_id), len(atom_id),
+                               _p(out))
     return out
+
+
+# Majorana spin-flip model (wavefunction.cu:75-237)
+def local_populations(p: Params, pos, psi_up, psi_dn, atom_id, pops=None):
+    """getLocalPopulations: (P_up, P_down) per label, shape (n_labels, 2)."""
+    atom_id = np.ascontiguousarray(atom_id, np.int32)
+    if pops is None:
+        pops = np.zeros((len(pos), 2))
+    load().orc_local_populations(C.byref(p), _p(pos), _p(psi_up), _p(psi_dn), _p(atom_id),
+                                 len(atom_id), _p(pops))
+    return pops
+
+
+def flip_atoms(p: Params, pos, vel, psi_up, psi_dn, pops, spin_up, atom_id, step):
+    """flipAtoms, in place on vel and spin_up; returns the number flipped."""
+    atom_id = np.ascontiguousarray(atom_id, np.int32)
+    assert vel.flags.c_contiguous and spin_up.dtype == np.uint8
+    return load().orc_flip_atoms(C.byref(p), _p(pos), _p(vel), _p(psi_up), _p(psi_dn), _p(pops),
+                                 _p(spin_up), _p(atom_id), len(atom_id), int(step))
+
+
+def decoherence_tau(p: Params, r):
+    return load().orc_decoherence_tau(C.byref(p), D3(*map(float, r)))
+
+
+def exponential_decay(p: Params, pos, psi_up, psi_dn, atom_id, dt):
+    atom_id = np.ascontiguousarray(atom_id, np.int32)
+    load().orc_exponential_decay(C.byref(p), _p(pos), _p(psi_up), _p(psi_dn), _p(atom_id),
+                                 len(atom_id), float(dt))
 
 
 def init_distribution(p: Params, n, T, T_axis=None, spin=False):
@@ -287,8 +327,9 @@ class Reference:
     """The reference's loop state (main.cu:88-140) on the CPU, single trap.
 
     Holds AoS state indexed by physical atom id plus the atomID list and
-    steps it exactly like main.cu:410-524 (without the dead median, MCWF and
-    output), using the oracle functions above."""
+    steps it exactly like main.cu:410-524 (without the dead median and the
+    output; the Majorana spin-flip model when run.mcwf), using the oracle
+    functions above."""
 
     def __init__(self, run, pos=None, vel=None, state=None):
         self.run = run
@@ -315,6 +356,9 @@ class Reference:
         self.cell = None
         self.evap_pos = np.zeros_like(self.s["pos"])
         self.evap_vel = np.zeros_like(self.s["vel"])
+        self.coll_steps = 0
+        self.flips = 0
+        self.pops = np.zeros((len(self.s["pos"]), 2))
 
     def _accel_all(self, state):
         # a(x) for every atom: one Verlet step with dt = 0 computes exactly
@@ -332,6 +376,10 @@ class Reference:
         counts, total = collide(self.p, self.s["vel"], self.sigvrmax, prefix, self.s["atom_id"],
                                 self.step_index)
         self.collisions += total
+        self.coll_steps += 1
+        if getattr(self.run, "mcwf", False):
+            local_populations(self.p, self.s["pos"], self.s["psi_up"], self.s["psi_dn"],
+                              self.s["atom_id"], self.pops)
         return dict(start_end=se, count=count, prefix=prefix, coll_count=counts, total=total)
 
     def motion_step(self):
@@ -355,6 +403,12 @@ class Reference:
                                         self.s.get("psi_dn") if run.spin_loss else None, 0.0,
                                         self.evap_pos, self.evap_vel)
             self.s["atom_id"] = remove_if(self.s["atom_id"], tag)
+        if getattr(run, "mcwf", False) and (self.step_index + 1) % run.lpc == 0:
+            self.flips += flip_atoms(self.p, self.s["pos"], self.s["vel"], self.s["psi_up"],
+                                     self.s["psi_dn"], self.pops, self.s["spin_up"],
+                                     self.s["atom_id"], self.coll_steps)
+            exponential_decay(self.p, self.s["pos"], self.s["psi_up"], self.s["psi_dn"],
+                              self.s["atom_id"], run.lpc * run.dt)
         if run.spin and (self.step_index + 1) % run.lpc == 0:
             normalise(self.s["psi_up"], self.s["psi_dn"], self.s["atom_id"])
         self.step_index += 1

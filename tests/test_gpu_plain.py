@@ -185,3 +185,80 @@ def test_golden_index_fixture(orc):
         assert lib.dsmc_remove_if(vals.ptr, sten.ptr, len(fx["sorted_id"]), C.byref(n_out), None) == 0
         got = vals.get()[: n_out.value]
     np.testing.assert_array_equal(got, fx["survivors"])
+
+
+def _tilted(orc, p, pos, q):
+    """Spinors with spin-down population q along the local field."""
+    n = len(pos)
+    u = np.zeros((n, 2)); d = np.zeros((n, 2))
+    for i in range(n):
+        B = orc.field(p, pos[i])
+        bx, by, bz = B / np.linalg.norm(B)
+        H = np.array([[bz, bx - 1j * by], [bx + 1j * by, -bz]])
+        w, V = np.linalg.eigh(H)
+        psi = np.sqrt(1 - q) * V[:, 1] + np.sqrt(q) * V[:, 0]
+        u[i] = psi[0].real, psi[0].imag
+        d[i] = psi[1].real, psi[1].imag
+    return u, d
+
+
+def test_spin_flip_calls_match_oracle(orc):
+    """dsmc_local_populations, dsmc_flip_atoms and dsmc_exponential_decay one
+    call at a time against the oracle: populations of aligned spins, then a
+    30 % spin-down admixture flips atoms (bit-exact velocities and spin
+    states), then decoherence over lpc * dt (spinors within 1e-13)."""
+    run = configs.reference(20, 20000).with_(mcwf=True)
+    n = run.n_atoms
+    p = orc.params_from(run)
+    st = orc.init_distribution(p, n, run.traps[0].T, spin=True)
+    ids = np.arange(n, dtype=np.int32)
+    print("spin-flip calls vs oracle, n=20000", flush=True)
+    with Simulation(run) as sim:
+        sim.set_state(st["pos"], st["vel"])
+        sim.set_spin(np.concatenate([st["psi_up"], st["psi_dn"]], axis=1))
+        sim.local_populations()
+        pops = orc.local_populations(p, st["pos"], st["psi_up"], st["psi_dn"], ids)
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_LOCAL_POPS), pops)
+        u, d = _tilted(orc, p, st["pos"], 0.3)
+        sim.set_spin(np.concatenate([u, d], axis=1))
+        vel = st["vel"].copy()
+        up = np.ones(n, np.uint8)
+        k_ref = orc.flip_atoms(p, st["pos"], vel, u, d, pops, up, ids, 0)
+        k = sim.flip_atoms()
+        assert k == k_ref and 0.25 * n < k < 0.35 * n
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_SPIN_UP), up)
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_VEL), vel)
+        sim.exponential_decay()
+        orc.exponential_decay(p, st["pos"], u, d, ids, run.lpc * run.dt)
+        sp = sim.copy_out(_abi.FIELD_SPIN)
+        np.testing.assert_allclose(sp[:, :2], u, rtol=0, atol=1e-13)
+        np.testing.assert_allclose(sp[:, 2:], d, rtol=0, atol=1e-13)
+
+
+def test_reference_run_with_spin_flips(orc):
+    """The reference loop with the spin-flip model (main.cu:457-521): 40
+    motion steps (4 collision steps) from spinors with a 2 % spin-down
+    admixture, against the oracle (positions, velocities and spin states
+    bit-exact)."""
+    run = configs.reference(20, 20000).with_(mcwf=True)
+    print("reference loop with spin flips", flush=True)
+    ref = orc.Reference(run)
+    u, d = _tilted(orc, ref.p, ref.s["pos"], 0.02)
+    ref.s["psi_up"][:], ref.s["psi_dn"][:] = u, d
+    with Simulation(run) as sim:
+        sim.set_state(ref.s["pos"], ref.s["vel"])
+        sim.set_spin(np.concatenate([u, d], axis=1))
+        sim.step(40)
+        ref.step(40)
+        ids = sim.copy_out(_abi.FIELD_ATOM_ID)
+        assert set(ids.tolist()) == set(ref.s["atom_id"].tolist())
+        alive = ref.s["atom_id"]
+        pos, _ = by_label(sim, _abi.FIELD_POS)
+        vel, _ = by_label(sim, _abi.FIELD_VEL)
+        np.testing.assert_array_equal(pos[alive], ref.s["pos"][alive])
+        np.testing.assert_array_equal(vel[alive], ref.s["vel"][alive])
+        up, _ = by_label(sim, _abi.FIELD_SPIN_UP)
+        np.testing.assert_array_equal(up[alive], ref.s["spin_up"][alive])
+        # flips are rare here (adiabatic following); the call-by-call test
+        # above exercises them in bulk
+        print(f"flips {ref.flips}, alive {len(alive)}", flush=True)

@@ -578,3 +578,129 @@ def test_run_loop_equals_step_composition(orc):
     np.testing.assert_array_equal(sv, ref.sigvrmax)
     assert res["collisions"] == ref.collisions and res["atom_steps"] == 6 * 3000
     del t
+
+
+# ---------------------------------------------------------------------------
+# Majorana spin-flip model (wavefunction.cu:75-237)
+# ---------------------------------------------------------------------------
+def _random_spinors(rng, n):
+    u = rng.normal(size=(n, 2)); d = rng.normal(size=(n, 2))
+    nrm = np.sqrt((u ** 2).sum(1) + (d ** 2).sum(1))[:, None]
+    return np.ascontiguousarray(u / nrm), np.ascontiguousarray(d / nrm)
+
+
+def _field_dirs(orc, p, pos):
+    B = np.array([orc.field(p, r) for r in pos])
+    return B / np.linalg.norm(B, axis=1)[:, None]
+
+
+def test_local_populations_are_projections_on_the_field(orc):
+    """getLocalPopulations: P_up = |<up_B|psi>|^2 with up_B the spin-up
+    eigenvector of B.sigma, computed here with complex 2x2 algebra; P_up +
+    P_down = 1 for a normalised spinor."""
+    run = configs.reference(20, 300)
+    p = orc.params_from(run)
+    st = orc.init_distribution(p, 300, 20e-6, spin=True)
+    rng = np.random.default_rng(11)
+    u, d = _random_spinors(rng, 300)
+    ids = np.arange(300, dtype=np.int32)
+    pops = orc.local_populations(p, st["pos"], u, d, ids)
+    Bn = _field_dirs(orc, p, st["pos"])
+    psi = np.stack([u[:, 0] + 1j * u[:, 1], d[:, 0] + 1j * d[:, 1]], axis=1)
+    for i in range(300):
+        bx, by, bz = Bn[i]
+        H = np.array([[bz, bx - 1j * by], [bx + 1j * by, -bz]])   # B.sigma
+        w, V = np.linalg.eigh(H)
+        up = V[:, np.argmax(w)]
+        assert pops[i, 0] == pytest.approx(abs(np.vdot(up, psi[i])) ** 2, abs=1e-13)
+    np.testing.assert_allclose(pops.sum(1), 1.0, atol=1e-14)
+
+
+def test_exponential_decay_is_partial_projection(orc):
+    """exponentialDecay: psi <- P+ psi + exp(-dt/tau) P- psi with P+- =
+    (1 +- B.sigma)/2 (matrix form here); dt = 0 is the identity and a very
+    long dt leaves the spin-up component along B."""
+    run = configs.reference(20, 200)
+    p = orc.params_from(run)
+    st = orc.init_distribution(p, 200, 20e-6, spin=True)
+    rng = np.random.default_rng(12)
+    ids = np.arange(200, dtype=np.int32)
+    Bn = _field_dirs(orc, p, st["pos"])
+    for dt in (0.0, 3e-6, 1e3):
+        u, d = _random_spinors(rng, 200)
+        psi = np.stack([u[:, 0] + 1j * u[:, 1], d[:, 0] + 1j * d[:, 1]], axis=1)
+        orc.exponential_decay(p, st["pos"], u, d, ids, dt)
+        for i in range(200):
+            bx, by, bz = Bn[i]
+            H = np.array([[bz, bx - 1j * by], [bx + 1j * by, -bz]])
+            Pp, Pm = (np.eye(2) + H) / 2, (np.eye(2) - H) / 2
+            e = math.exp(-dt / orc.decoherence_tau(p, st["pos"][i]))
+            want = Pp @ psi[i] + e * (Pm @ psi[i])
+            got = np.array([u[i, 0] + 1j * u[i, 1], d[i, 0] + 1j * d[i, 1]])
+            np.testing.assert_allclose(got, want, atol=1e-13)
+            if dt == 0.0:
+                np.testing.assert_allclose(got, psi[i], atol=1e-15)
+
+
+def test_decoherence_time_from_the_field_gradient(orc):
+    """calculateTau with getAcc: the relative acceleration 2 |mu grad|B|| / m is
+    taken here from a finite difference of |B|; tau is the cube-harmonic mean of
+    1.163 sqrt(sigma / a) and sqrt(2 / pi) hbar / (m sigma a)."""
+    run = configs.reference(20, 50)
+    p = orc.params_from(run)
+    st = orc.init_distribution(p, 50, 20e-6)
+    h = 1e-9
+    for r in st["pos"][:20]:
+        g = np.zeros(3)
+        for k in range(3):
+            e = np.zeros(3); e[k] = h
+            g[k] = (np.linalg.norm(orc.field(p, r + e)) - np.linalg.norm(orc.field(p, r - e))) / (2 * h)
+        a = 2.0 * p.mu * np.linalg.norm(g) / p.mass
+        tp = 1.163 * math.sqrt(p.sigma / a)
+        tv = math.sqrt(2.0 / configs.PI_REF) * configs.HBAR / (p.mass * p.sigma * a)
+        want = (tp ** -3 + tv ** -3) ** (-1.0 / 3.0)
+        assert orc.decoherence_tau(p, r) == pytest.approx(want, rel=1e-6)
+
+
+def test_flips_conserve_energy_and_follow_the_population_change(orc):
+    """flipAtoms: an atom whose spin-down population grew by q since the
+    populations were taken flips with probability q (binomial, 5 sigma); a
+    flip keeps the velocity direction and conserves E_k + U, with U_down =
+    -U_up - 2 mu B0 (a potential step of 2 mu |B|)."""
+    run = configs.reference(20, 20000)
+    p = orc.params_from(run)
+    n = 20000
+    st = orc.init_distribution(p, n, 20e-6, spin=True)
+    ids = np.arange(n, dtype=np.int32)
+    pops = orc.local_populations(p, st["pos"], st["psi_up"], st["psi_dn"], ids)
+    np.testing.assert_allclose(pops[:, 0], 1.0, atol=1e-12)   # aligned spin up
+    # rotate every spinor so that its spin-down population becomes q:
+    # psi = sqrt(1-q) up_B + sqrt(q) down_B
+    q = 0.3
+    Bn = _field_dirs(orc, p, st["pos"])
+    u = np.zeros((n, 2)); d = np.zeros((n, 2))
+    for i in range(n):
+        bx, by, bz = Bn[i]
+        H = np.array([[bz, bx - 1j * by], [bx + 1j * by, -bz]])
+        w, V = np.linalg.eigh(H)
+        psi = math.sqrt(1 - q) * V[:, 1] + math.sqrt(q) * V[:, 0]
+        u[i] = psi[0].real, psi[0].imag
+        d[i] = psi[1].real, psi[1].imag
+    vel = st["vel"] * 10.0   # kinetic energy far above the potential step
+    v0 = vel.copy()
+    up = np.ones(n, np.uint8)
+    e0 = orc.energy(p, st["pos"], vel, ids, up)
+    k = orc.flip_atoms(p, st["pos"], vel, u, d, pops, up, ids, 7)
+    assert abs(k - q * n) < 5 * math.sqrt(n * q * (1 - q))
+    flipped = up == 0
+    assert flipped.sum() == k
+    e1 = orc.energy(p, st["pos"], vel, ids, up)
+    assert (e1[0] + e1[1]) == pytest.approx(e0[0] + e0[1], rel=1e-12)
+    cosang = (vel * v0).sum(1) / np.linalg.norm(vel, axis=1) / np.linalg.norm(v0, axis=1)
+    np.testing.assert_allclose(cosang, 1.0, atol=1e-12)
+    np.testing.assert_array_equal(vel[~flipped], v0[~flipped])
+    # the same draws again flip exactly the same atoms
+    up2 = np.ones(n, np.uint8)
+    vel2 = v0.copy()
+    assert orc.flip_atoms(p, st["pos"], vel2, u, d, pops, up2, ids, 7) == k
+    np.testing.assert_array_equal(up2, up)
