@@ -8,8 +8,9 @@ Named workloads
 ---------------
 reference(Nc, N)  the reference's own run (main.cu:45-69, 165-168): Ioffe-Pritchard
                   trap, moment 0.5*gs*muB, 20 uK, grid +-1 mm, dt = 1e-7 s,
-                  10 motion substeps per collision step, spinor evolution and
-                  spin-projection loss, alpha = 3e6 / N, its biased pair choice.
+                  10 motion substeps per collision step, spinor evolution,
+                  spin-projection loss and the Majorana spin-flip model of
+                  main.cu:457-521, alpha = 3e6 / N, its biased pair choice.
 quadrupole_1e4    N = 1e4 in a spherical quadrupole, B' = 2.5 T/m, moment muB/2,
                   20 uK, 35^3 cells over +-5 thermal radii, dt = 1e-6 s, FP64,
                   no spin, no evaporation (the CPU-runnable case).
@@ -155,7 +156,7 @@ def reference(Nc: int = 50, N: int = 10000) -> Run:
     trap = Trap(kind="ip", n_atoms=N, nc=(Nc, Nc, Nc), half_width=(1e-3, 1e-3, 1e-3),
                 B0=0.01, dBdx=20., d2Bdx2=40000., mu=0.5 * GS * MUB, T=20e-6, n_real=N_REAL_REF)
     return Run(traps=[trap], precision="fp64", pair_mode="reference", spin=True, spin_substeps=1,
-               spin_interp=False, spin_loss=True, lpc=10, dt=1e-7, seed=1234,
+               spin_interp=False, spin_loss=True, mcwf=True, lpc=10, dt=1e-7, seed=1234,
                steps=int(math.ceil(5e-3 / 1e-7)), name=f"reference-{Nc}-{N}")
 
 

@@ -131,6 +131,7 @@ int main(int argc, const char *argv[])
         cfg.evaporation = DSMC_EVAP_SPIN;
         cfg.spin = 1;
         cfg.spin_substeps = 1;
+        cfg.mcwf = 1;   // getLocalPopulations / flipAtoms / exponentialDecay (main.cu:457-516)
         cfg.lpc = 10;
         cfg.dt = 1e-7;
         cfg.seed = 1234;

@@ -377,7 +377,7 @@ class Reference:
                                 self.step_index)
         self.collisions += total
         self.coll_steps += 1
-        if getattr(self.run, "mcwf", False):
+        if getattr(self.run, "mcwf", False) and self.run.spin:
             local_populations(self.p, self.s["pos"], self.s["psi_up"], self.s["psi_dn"],
                               self.s["atom_id"], self.pops)
         return dict(start_end=se, count=count, prefix=prefix, coll_count=counts, total=total)
@@ -403,7 +403,7 @@ class Reference:
                                         self.s.get("psi_dn") if run.spin_loss else None, 0.0,
                                         self.evap_pos, self.evap_vel)
             self.s["atom_id"] = remove_if(self.s["atom_id"], tag)
-        if getattr(run, "mcwf", False) and (self.step_index + 1) % run.lpc == 0:
+        if getattr(run, "mcwf", False) and run.spin and (self.step_index + 1) % run.lpc == 0:
             self.flips += flip_atoms(self.p, self.s["pos"], self.s["vel"], self.s["psi_up"],
                                      self.s["psi_dn"], self.pops, self.s["spin_up"],
                                      self.s["atom_id"], self.coll_steps)

@@ -88,6 +88,8 @@ def _create(run):
      _abi.DSMC_EINVAL, "quadrupole"),
     (lambda r: r.with_(traps=[configs.Trap(n_atoms=10, nc=(0, 4, 4))]), _abi.DSMC_EINVAL, "nc"),
     (lambda r: r.with_(obs_capacity=-1), _abi.DSMC_EINVAL, "obs_capacity"),
+    (lambda r: r.with_(mcwf=True), _abi.DSMC_ENOTSUP, "spin-flip"),
+    (lambda r: r.with_(mcwf=True, spin=True, precision="fp32"), _abi.DSMC_ENOTSUP, "spin-flip"),
 ])
 def test_invalid_configuration_rejected(mutate, status, needle):
     st, msg = _create(mutate(configs.quadrupole_1e4()))

@@ -86,7 +86,7 @@ def test_trajectory_bit_exact(orc, kind):
     if kind == "quadrupole":
         run = quad_run(n=20000, nc=30)
     elif kind == "ip":
-        run = configs.reference(20, 20000).with_(spin=False, spin_loss=False, lpc=1, dt=1e-6,
+        run = configs.reference(20, 20000).with_(spin=False, spin_loss=False, mcwf=False, lpc=1, dt=1e-6,
                                                  pair_mode="uniform")
     else:
         run = configs.box_thermalise("fp64", n_atoms=30000, nc=24)
