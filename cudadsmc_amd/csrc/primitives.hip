@@ -380,44 +380,47 @@ __global__ __launch_bounds__(256) void k_ins_locate(
     const int *__restrict__ counts, uint32_t *__restrict__ mov_k, uint32_t *__restrict__ mov_q,
     uint32_t *__restrict__ mov_r, uint32_t *__restrict__ bcnt)
 {
-    const int j = blockIdx.x * 256 + threadIdx.x;
     const int s = *n_dev, m = counts[1];
     if (blockIdx.x * 256 >= m) return;   // whole block leaves together
-    const bool valid = j < m;
-    const uint32_t v = valid ? mov_v[j] : 0u;
-    const uint32_t k = valid ? key_new[v] : 0u, k0 = valid ? key_old[v] : 0u;
-    const int p0 = (int)v;
-    int lo, hi;   // the insertion point lies in [lo, hi]
-    if (!valid) {
-        lo = hi = 0;
-    } else if (k0 < k) {   // (key_old[p0], p0) < (k, v): key_old[p0] != k
-        int off = 1;
-        while (p0 + off < s && kv_less(key_old[p0 + off], (uint32_t)(p0 + off), k, v)) off <<= 1;
-        lo = p0 + (off >> 1) + 1;
-        hi = min(p0 + off, s);
-    } else {
-        int off = 1;
-        while (p0 - off >= 0 && !kv_less(key_old[p0 - off], (uint32_t)(p0 - off), k, v)) off <<= 1;
-        lo = p0 - off < 0 ? 0 : p0 - off + 1;
-        hi = p0 - (off >> 1);
-    }
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (kv_less(key_old[mid], (uint32_t)mid, k, v)) lo = mid + 1; else hi = mid;
-    }
-    // bucket slot: lanes of a wave with the same bucket share one atomic
-    // (nearby movers mostly share buckets)
-    const uint32_t bkt = (uint32_t)lo >> kStayShift;
-    const uint64_t peers = match_bits<kBucketBits>(bkt, __ballot(valid));
-    const int leader = __ffsll((unsigned long long)peers) - 1;
-    uint32_t base = 0;
-    if (valid && (int)(threadIdx.x & (kWave - 1)) == leader)
-        base = atomicAdd(&bcnt[bkt], (uint32_t)__popcll(peers));
-    base = __shfl(base, leader < 0 ? 0 : leader, kWave);
-    if (valid) {
-        mov_k[j] = k;
-        mov_q[j] = (uint32_t)lo;
-        mov_r[j] = base + (uint32_t)__popcll(peers & lanemask_lt());
+    // the grid covers 1/16 of the slots; more movers take further rounds
+    for (int j0 = 0; j0 < m; j0 += (int)gridDim.x * 256) {
+        const int j = j0 + blockIdx.x * 256 + threadIdx.x;
+        const bool valid = j < m;
+        const uint32_t v = valid ? mov_v[j] : 0u;
+        const uint32_t k = valid ? key_new[v] : 0u, k0 = valid ? key_old[v] : 0u;
+        const int p0 = (int)v;
+        int lo, hi;   // the insertion point lies in [lo, hi]
+        if (!valid) {
+            lo = hi = 0;
+        } else if (k0 < k) {   // (key_old[p0], p0) < (k, v): key_old[p0] != k
+            int off = 1;
+            while (p0 + off < s && kv_less(key_old[p0 + off], (uint32_t)(p0 + off), k, v)) off <<= 1;
+            lo = p0 + (off >> 1) + 1;
+            hi = min(p0 + off, s);
+        } else {
+            int off = 1;
+            while (p0 - off >= 0 && !kv_less(key_old[p0 - off], (uint32_t)(p0 - off), k, v)) off <<= 1;
+            lo = p0 - off < 0 ? 0 : p0 - off + 1;
+            hi = p0 - (off >> 1);
+        }
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (kv_less(key_old[mid], (uint32_t)mid, k, v)) lo = mid + 1; else hi = mid;
+        }
+        // bucket slot: lanes of a wave with the same bucket share one atomic
+        // (nearby movers mostly share buckets)
+        const uint32_t bkt = (uint32_t)lo >> kStayShift;
+        const uint64_t peers = match_bits<kBucketBits>(bkt, __ballot(valid));
+        const int leader = __ffsll((unsigned long long)peers) - 1;
+        uint32_t base = 0;
+        if (valid && (int)(threadIdx.x & (kWave - 1)) == leader)
+            base = atomicAdd(&bcnt[bkt], (uint32_t)__popcll(peers));
+        base = __shfl(base, leader < 0 ? 0 : leader, kWave);
+        if (valid) {
+            mov_k[j] = k;
+            mov_q[j] = (uint32_t)lo;
+            mov_r[j] = base + (uint32_t)__popcll(peers & lanemask_lt());
+        }
     }
 }
 
@@ -727,7 +730,8 @@ void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const 
     DSMC_LAUNCH(prof, "resort_collect", 0.0, k_ins_collect, div_up(words, 256), 256, stream,
                 mask_m, mask_d, n_dev, dead_key, r.tile_cnt, r.tile_dead, r.counts, r.mov_v,
                 out_k, out_v, r.bcnt);
-    DSMC_LAUNCH(prof, "resort_locate", 0.0, k_ins_locate, div_up(r.n_cap, 256), 256, stream,
+    DSMC_LAUNCH(prof, "resort_locate", 0.0, k_ins_locate, div_up(div_up(r.n_cap, 16), 256), 256,
+                stream,
                 key_new, key_old, r.mov_v, n_dev, r.counts, r.mov_k, r.mov_q, r.mov_r, r.bcnt);
     DSMC_LAUNCH(prof, "resort_place", 0.0, k_ins_place, kPlaceBlocks, 256, stream, r.mov_k,
                 r.mov_v, r.mov_q, r.mov_r, n_dev, r.counts, r.bcnt, r.bk_k, r.bk_v, r.bk_q, r.bex);
