@@ -384,19 +384,44 @@ __device__ __forceinline__ void scatter_pair(const CollideParams &cp, uint32_t s
     *zb = f4{ nb.vz, nb.rvx, nb.rvy, nb.rvz };
 }
 
+// The first trial of a cell (its pair, acceptance uniform and relative
+// speed), loaded before the cell's trial count is known: the count needs
+// sigma v_r,max, and the two loads then overlap.
+struct Trial0 {
+    uint32_t a, b;
+    float u, vr;
+};
+
+__device__ __forceinline__ void trial0_load(const CollideParams &cp, uint32_t stream, uint32_t cell,
+                                            uint32_t step, int n, int i,
+                                            const uint32_t *__restrict__ perm, const FusedPtrs &s,
+                                            Trial0 &t)
+{
+    int ia, ib;
+    trial_draw(cp, stream, cell, step, 0, n, ia, ib, t.u);
+    t.a = perm[i + ia];
+    t.b = perm[i + ib];
+    t.vr = rel_speed(s, t.a, t.b);
+}
+
 // The trials of one cell in sequence, by one lane (collisions.cu:397-441).
 __device__ __forceinline__ int cell_trials_serial(const CollideParams &cp, const CellJob &c,
                                                   uint32_t stream, uint32_t step,
                                                   const uint32_t *__restrict__ perm,
-                                                  const FusedPtrs &s, double &smax)
+                                                  const FusedPtrs &s, double &smax,
+                                                  const Trial0 &t0)
 {
     int count = 0;
     for (int tr = 0; tr < c.ncol; ++tr) {
-        int ia, ib;
-        float u;
-        trial_draw(cp, stream, c.cell, step, tr, c.n, ia, ib, u);
-        const uint32_t a = perm[c.i + ia], b = perm[c.i + ib];
-        const float vr = rel_speed(s, a, b);
+        uint32_t a = t0.a, b = t0.b;
+        float u = t0.u, vr = t0.vr;
+        if (tr > 0) {
+            int ia, ib;
+            trial_draw(cp, stream, c.cell, step, tr, c.n, ia, ib, u);
+            a = perm[c.i + ia];
+            b = perm[c.i + ib];
+            vr = rel_speed(s, a, b);
+        }
         const double sv = (double)vr * cp.sigma;
         if (sv > smax) smax = sv;
         if (c.pc * vr > u) {
@@ -462,11 +487,11 @@ constexpr int kWaveTrialsDefault = 16;
 __device__ __forceinline__ int run_cell(const CollideParams &cp, const CellJob &c, uint32_t stream,
                                         uint32_t step, const uint32_t *__restrict__ perm,
                                         const FusedPtrs &s, double *__restrict__ sigvrmax,
-                                        bool wave)
+                                        bool wave, const Trial0 &t0)
 {
     double smax = c.smax;
     const int cnt = wave ? cell_trials_wave(cp, c, stream, step, perm, s, smax)
-                         : cell_trials_serial(cp, c, stream, step, perm, s, smax);
+                         : cell_trials_serial(cp, c, stream, step, perm, s, smax, t0);
     const bool writer = !wave || (threadIdx.x & (kWave - 1)) == 0;
     if (writer && smax != c.smax) sigvrmax[c.k] = smax;
     return cnt;
@@ -492,23 +517,41 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     __shared__ int heads[kFBlock / kWave][kChunk + 1];
     __shared__ short multi[kFBlock / kWave][kChunk];   // heads index of cells with >= 2 atoms
     __shared__ short heavy[kFBlock / kWave][kChunk];
-    load_table(kb, key_base, T + 1);
     const int n = *n_dev;
     const uint32_t step = (uint32_t)*step_dev;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int chunk0 = (blockIdx.x * (kFBlock / kWave) + w) * kChunk;
     const int chunk1 = min(n, chunk0 + kChunk);
     const uint64_t lt = lanemask_lt();
-    int nh = 0, nm = 0;
-    uint32_t klast = 0;   // key of the chunk's last cell head
+    // the chunk's keys and the two beyond its ends (kNoKey: none), loaded
+    // before the trap table so that the loads overlap
+    constexpr uint32_t kNoKey = 0xFFFFFFFFu;   // keys are below 2^31
+    uint32_t kk[kCollItems];
+#pragma unroll
     for (int it = 0; it < kCollItems; ++it) {
         const int i = chunk0 + it * kWave + lane;
+        kk[it] = i < n ? key[i] : kNoKey;
+    }
+    const uint32_t k_before = chunk0 > 0 && chunk0 <= n ? key[chunk0 - 1] : kNoKey;
+    const uint32_t k_after = chunk0 + kChunk < n ? key[chunk0 + kChunk] : kNoKey;
+    load_table(kb, key_base, T + 1);
+    int nh = 0, nm = 0;
+    uint32_t klast = 0;   // key of the chunk's last cell head
+#pragma unroll
+    for (int it = 0; it < kCollItems; ++it) {
+        const int i = chunk0 + it * kWave + lane;
+        const uint32_t k = kk[it];
+        // neighbours from the adjacent lanes (and items)
+        uint32_t kp = __shfl_up(k, 1, kWave), kn = __shfl_down(k, 1, kWave);
+        const uint32_t prev_last = it > 0 ? __shfl(kk[it > 0 ? it - 1 : 0], kWave - 1, kWave) : k_before;
+        const uint32_t next_first = it + 1 < kCollItems ? __shfl(kk[it + 1 < kCollItems ? it + 1 : it], 0, kWave)
+                                                        : k_after;
+        if (lane == 0) kp = prev_last;
+        if (lane == kWave - 1) kn = next_first;
         bool head = false, mult = false;
-        uint32_t k = 0;
         if (i < n) {
-            k = key[i];
-            head = i == 0 || key[i - 1] != k;
-            mult = head && i + 1 < n && key[i + 1] == k;
+            head = i == 0 || kp != k;
+            mult = head && i + 1 < n && kn == k;
         }
         const uint64_t bh = __ballot(head), bm = __ballot(mult);
         const int pos = nh + __popcll(bh & lt);
@@ -553,17 +596,24 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     for (int m0 = 0; m0 < nm; m0 += kWave) {
         const int mi = m0 + lane;
         CellJob c;
+        Trial0 t0;
         bool ok = false;
         int h = 0;
+        uint32_t stream = 0;
         if (mi < nm) {
             h = multi[w][mi];
-            const int i = heads[w][h];
+            const int i = heads[w][h], end = heads[w][h + 1];
             const uint32_t k = key[i];
+            auto setup = [&](const TrapDev &tr, int t) {
+                stream = tr.stream;
+                trial0_load(cp, stream, k - (uint32_t)tr.key_base, step, end - i, i, perm, s, t0);
+                ok = cell_setup(tr, t, k, cp, sigvrmax, i, end, c);
+            };
             if (t_lo == t_hi) {
-                ok = cell_setup(traps[t_lo], t_lo, k, cp, sigvrmax, i, heads[w][h + 1], c);
+                setup(traps[t_lo], t_lo);
             } else {
                 const int t = last_le(kb, T + 1, (int)k);
-                ok = cell_setup(traps[t], t, k, cp, sigvrmax, i, heads[w][h + 1], c);
+                setup(traps[t], t);
             }
         }
         const bool is_heavy = ok && c.ncol > wave_trials;
@@ -571,7 +621,7 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
         if (is_heavy) heavy[w][nheavy + __popcll(hb & lt)] = (short)mi;
         nheavy += __popcll(hb);
         if (ok && !is_heavy) {
-            const int cnt = run_cell(cp, c, traps[c.t].stream, step, perm, s, sigvrmax, false);
+            const int cnt = run_cell(cp, c, stream, step, perm, s, sigvrmax, false, t0);
             if (t_lo == t_hi) wave_cnt += (unsigned long long)cnt;
             else if (cnt) atomicAdd(&coll_accum[c.t], (unsigned long long)cnt);
         }
@@ -584,7 +634,8 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
         const int t = t_lo == t_hi ? t_lo : last_le(kb, T + 1, (int)k);
         CellJob c;
         (void)cell_setup(traps[t], t, k, cp, sigvrmax, i, heads[w][h + 1], c);
-        const int cnt = run_cell(cp, c, traps[t].stream, step, perm, s, sigvrmax, true);
+        const Trial0 none = { 0u, 0u, 0.0f, 0.0f };
+        const int cnt = run_cell(cp, c, traps[t].stream, step, perm, s, sigvrmax, true, none);
         if (lane == 0) {
             if (t_lo == t_hi) wave_cnt += (unsigned long long)cnt;
             else if (cnt) atomicAdd(&coll_accum[t], (unsigned long long)cnt);
