@@ -38,9 +38,9 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per atom (or per pass) of each kernel of the FP32 form.
-#   move_fused: read perm (4) + 12 floats of state (48) + label (4); write
-#               12 floats (48) + label (4) + next cell key (4).  The key the
-#               atom had before the move is recomputed from its position.
+#   move_fused: read perm (4) + 12 floats of state (48) + label (4) + the
+#               last sort's key of the slot (4); write 12 floats (48) + label
+#               (4) + next cell key (4)
 #   resort_split: read new and previous key (8), write the unchanged atoms'
 #               (key, slot) (8); resort_merge: read those (8), write the sorted
 #               (key, slot) (8) -- movers (a few percent) not counted
@@ -49,7 +49,7 @@ HBM_PEAK_GBS = 8000.0
 #   table_fused: read key; collide_fused: read key (per slot) -- per-cell work
 #               is counted by the measured traffic, not here.
 PER_ATOM_BYTES = {
-    "move_fused": 112.0,
+    "move_fused": 116.0,
     "resort_split": 16.0,
     "resort_merge": 16.0,
     "radix_downsweep": 16.0,

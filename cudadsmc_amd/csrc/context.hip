@@ -658,7 +658,10 @@ static void fused_move(dsmc_ctx *x)
     MoveParams mp = move_params(x);
     // a sorted move also counts, per move block, the atoms whose key changed:
     // the input of the next step's incremental re-sort
-    if (x->have_sort) mp.count_moved = 1;
+    if (x->have_sort) {
+        mp.count_moved = 1;
+        mp.key_prev = x->sorted_key;
+    }
     launch_move_fused(x->block, x->apt, x->d_traps, mp, x->fs[x->cur], x->fs[x->cur ^ 1],
                       x->have_sort ? x->perm : nullptr, x->trap_start, n, x->key_a,
                       x->evap_pos_f, x->evap_vel_f, x->rec, x->step_dev, x->n_cap, x->stream,

@@ -661,7 +661,7 @@ struct MoveAtom {
     float sx, sy, sz;
     int id;
     int t;          // trap
-    uint32_t kold;  // key before the move (when counting)
+    uint32_t kold;  // key of the last sort (before the move), when counting
 };
 
 __device__ __forceinline__ void move_load(const MoveParams &mp, const FusedPtrs &A,
@@ -671,13 +671,7 @@ __device__ __forceinline__ void move_load(const MoveParams &mp, const FusedPtrs 
     m.id = ld_at(A.id, j);
     m.sx = m.sy = m.sz = 0.0f;
     if (mp.spin) { m.sx = ld_at(A.s[0], j); m.sy = ld_at(A.s[1], j); m.sz = ld_at(A.s[2], j); }
-    // key of the slot before the move: the stored position gives exactly the
-    // key the last sort used (recomputed instead of read)
     m.kold = 0;
-    if (mp.count_moved)
-        m.kold = (uint32_t)(tr.key_base + cell_of_fast(tr, (double)m.f.px + (double)m.f.rpx,
-                                                       (double)m.f.py + (double)m.f.rpy,
-                                                       (double)m.f.pz + (double)m.f.rpz));
 }
 
 // Spin substeps, Kahan velocity Verlet, wrap, evaporation test, next key;
@@ -824,7 +818,10 @@ __global__ __launch_bounds__(BS) void k_move_fused(
         for (int k = 0; k < APT; ++k) {
             const int i = b0 + k * BS + (int)threadIdx.x;
             at[k].t = i < n ? t_first : -1;
-            if (i < n) move_load(mp, A, tr, perm ? ld_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
+            if (i < n) {
+                move_load(mp, A, tr, perm ? ld_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
+                if (mp.count_moved) at[k].kold = ld_at(mp.key_prev, (uint32_t)i);
+            }
         }
 #pragma unroll
         for (int k = 0; k < APT; ++k) {
@@ -843,6 +840,7 @@ __global__ __launch_bounds__(BS) void k_move_fused(
                 while (t < t_last && ts[t + 1] <= i) ++t;
                 at[k].t = t;
                 move_load(mp, A, traps[t], perm ? ld_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
+                if (mp.count_moved) at[k].kold = ld_at(mp.key_prev, (uint32_t)i);
             }
         }
 #pragma unroll

@@ -84,10 +84,11 @@ struct MoveParams {
     int    obs_cap;      // records in the observables ring (per trap)
     uint32_t dead_key;
     // when count_moved is set: per move block, the atoms whose new key differs
-    // from their key before the move (the last sort's key of the slot) and the
-    // removed atoms, and the same atoms as one bit per slot (64-slot words),
-    // for the incremental re-sort of the next step
+    // from their key before the move (key_prev: the last sort's sorted keys,
+    // indexed by output slot) and the removed atoms, and the same atoms as one
+    // bit per slot (64-slot words), for the incremental re-sort of the next step
     int count_moved;
+    const uint32_t *key_prev;
     uint32_t *blk_moved;
     uint32_t *blk_dead;
     uint64_t *mask_moved;
