@@ -960,7 +960,8 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
             }
         }
     }
-    block_sum<256, kAccK>(a, red);
+    static_assert(kAccK == 8, "eight accumulators");
+    block_sum8<256>(a, red);
     // one block per trap, or the last of the trap's P blocks, finalizes it
     double f[kAccK];
     if (P == 1) {
