@@ -159,6 +159,7 @@ struct dsmc_ctx {
     uint32_t *cell_start = nullptr, *cell_end = nullptr;
     uint32_t *blk_moved = nullptr, *blk_dead = nullptr;   // per move block, for the re-sort
     uint64_t *mask_moved = nullptr, *mask_dead = nullptr; // per 64 slots, for the re-sort
+    uint32_t *key_samp = nullptr, *key_tsamp = nullptr;   // samples of the last sort's keys
     int *trap_start = nullptr;
     double *acc = nullptr, *acc2 = nullptr;
     ObsRecords rec{};
@@ -390,6 +391,8 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
         x->blk_dead = x->alloc<uint32_t>((size_t)div_up(N, 2048) * (2048 / 64) + 64);
         x->mask_moved = x->alloc<uint64_t>(resort_mask_words(N));
         x->mask_dead = x->alloc<uint64_t>(resort_mask_words(N));
+        x->key_samp = x->alloc<uint32_t>(resort_mask_words(N));
+        x->key_tsamp = x->alloc<uint32_t>(resort_mask_words(N) / (kResortTile / 64) + 1);
         x->srt_k[0] = x->key_b; x->srt_v[0] = x->val_b;
         x->srt_k[1] = x->alloc<uint32_t>(N); x->srt_v[1] = x->alloc<uint32_t>(N);
         x->cell_start = x->alloc<uint32_t>(x->n_keys, false);
@@ -593,6 +596,8 @@ static MoveParams move_params(const dsmc_ctx *x)
     mp.blk_dead = x->blk_dead;
     mp.mask_moved = x->mask_moved;
     mp.mask_dead = x->mask_dead;
+    mp.key_samp = x->key_samp;
+    mp.key_tsamp = x->key_tsamp;
     return mp;
 }
 
@@ -613,7 +618,8 @@ static void fused_sort(dsmc_ctx *x)
     uint32_t *ok = x->srt_k[x->srt_cur ^ 1], *ov = x->srt_v[x->srt_cur ^ 1];
     if (x->srt_valid && x->cfg.sort_mode == DSMC_SORT_INCREMENTAL && resort_supported(x->n_cap))
         resort_incremental(x->key_a, x->srt_k[x->srt_cur], n, x->dead_key, x->blk_moved,
-                           x->blk_dead, x->mask_moved, x->mask_dead, x->block * x->apt, x->res,
+                           x->blk_dead, x->mask_moved, x->mask_dead, x->key_samp, x->key_tsamp,
+                           x->block * x->apt, x->res,
                            ok, ov, x->stream, &x->prof);
     else
         radix_sort(x->key_a, nullptr, x->res.alt_k, x->res.alt_v, ok, ov, n, x->key_bits,

@@ -91,7 +91,9 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 
 // Incremental re-sort (see primitives.hip): the stable sort of key_new[0..n)
 // by (key, slot), for slots that are in stable sorted order of key_old, into
-// out (which must not alias key_old).  dead_key sorts last.
+// out (which must not alias key_old).  dead_key sorts last.  Slots are
+// grouped in tiles of kResortTile.
+constexpr int kResortTile = 2048;
 struct ResortScratch {
     int n_cap = 0, tiles = 0;
     uint32_t *mov_k = nullptr, *mov_v = nullptr;
@@ -108,11 +110,14 @@ bool resort_supported(int n_cap);
 size_t resort_mask_words(int n_cap);
 // blk_moved / blk_dead: per block of move_block slots, the atoms whose key
 // changed (not removed) and the removed atoms; mask_m / mask_d: the same
-// atoms as one bit per slot (written by the fused move).
+// atoms as one bit per slot; key_samp / key_tsamp: key_old[64 w] for every
+// 64-slot word w and key_old[kResortTile t] for every tile t below n (all
+// written by the fused move).
 void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const int *n_dev,
                         uint32_t dead_key, const uint32_t *blk_moved, const uint32_t *blk_dead,
-                        const uint64_t *mask_m, const uint64_t *mask_d, int move_block, const ResortScratch &r, uint32_t *out_k, uint32_t *out_v,
-                        hipStream_t stream, Profiler *prof);
+                        const uint64_t *mask_m, const uint64_t *mask_d, const uint32_t *key_samp,
+                        const uint32_t *key_tsamp, int move_block, const ResortScratch &r,
+                        uint32_t *out_k, uint32_t *out_v, hipStream_t stream, Profiler *prof);
 
 // Exclusive scan of n ints (n known on the host); partial holds div_up(n, tile).
 void exclusive_scan(const int *in, int *out, int n, int *partial, hipStream_t stream,

@@ -870,6 +870,10 @@ __global__ __launch_bounds__(BS) void k_move_fused(
                 const int wd = (b0 + k * BS) / kWave + (int)threadIdx.x / kWave;
                 mp.mask_moved[wd] = bm;
                 mp.mask_dead[wd] = bd;
+                if (at[k].t >= 0) {   // slot 64 wd is below n
+                    mp.key_samp[wd] = at[k].kold;
+                    if (wd % (kResortTile / kWave) == 0) mp.key_tsamp[wd / (kResortTile / kWave)] = at[k].kold;
+                }
             }
         }
         if ((threadIdx.x & (kWave - 1)) == 0) {

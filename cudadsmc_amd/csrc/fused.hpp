@@ -86,13 +86,17 @@ struct MoveParams {
     // when count_moved is set: per move block, the atoms whose new key differs
     // from their key before the move (key_prev: the last sort's sorted keys,
     // indexed by output slot) and the removed atoms, and the same atoms as one
-    // bit per slot (64-slot words), for the incremental re-sort of the next step
+    // bit per slot (64-slot words), for the incremental re-sort of the next step;
+    // key_samp / key_tsamp receive key_prev at every 64th and every
+    // kResortTile-th slot below n (the re-sort's search index)
     int count_moved;
     const uint32_t *key_prev;
     uint32_t *blk_moved;
     uint32_t *blk_dead;
     uint64_t *mask_moved;
     uint64_t *mask_dead;
+    uint32_t *key_samp;
+    uint32_t *key_tsamp;
 };
 
 struct TableParams {

@@ -212,8 +212,9 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 //      written straight to the end of the output, in slot order;
 //   3. locate: each mover's insertion point q in slot space, the first slot
 //      j with (key_old[j], j) >= (key, slot) (binary search of key_old, whose
-//      stays equal key_new): the mover goes after every stay below q and
-//      before every stay from q on.  Movers are bucketed by the tile of q;
+//      stays equal key_new, through samples of it that the move kernel
+//      left): the mover goes after every stay below q and before every stay
+//      from q on.  Movers are bucketed by the tile of q;
 //   4. place: movers are copied into their bucket;
 //   5. merge: one block per tile ranks its bucket's movers by counting
 //      (buckets hold a few dozen), and writes the tile's stays and movers to
@@ -223,7 +224,7 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 //      of the tile below q) + r.
 // No kernel waits on another block.
 // ---------------------------------------------------------------------------
-constexpr int kStayTile = 2048;                // slots per tile (and bucket)
+constexpr int kStayTile = kResortTile;         // slots per tile (and bucket)
 constexpr int kStayShift = 11;
 constexpr int kTileWords = kStayTile / 64;     // mask words per tile
 constexpr int kBucketLds = 1024;               // movers a bucket ranks in LDS
@@ -371,41 +372,59 @@ __device__ __forceinline__ bool kv_less(uint32_t ka, uint32_t va, uint32_t kb, u
 
 // Insertion point q of every mover in slot space: the first slot j with
 // (key_old[j], j) >= (key, slot) (key_old is sorted, so this is a prefix).
-// The search gallops from the mover's own slot: a move to the next cell
-// along x lands within a few places, along y or z within about Nc or Nc^2
-// cells' worth of atoms.  Also the mover's place in q's bucket.
+// The move kernel sampled key_old at every tile start (tsamp, 2048 slots
+// apart) and every 64 slots (samp).  The search narrows q to one tile by a
+// binary search of the tile samples in LDS, to 64 slots by the (at most 32)
+// samples of that tile, and then searches those 64 slots of key_old: at most
+// about 24 dependent steps, 13 of them in LDS and the rest within four cache
+// lines, wherever the mover lands (a move along z crosses about Nc^2 cells'
+// worth of atoms).  Also the mover's place in q's bucket.
 __global__ __launch_bounds__(256) void k_ins_locate(
     const uint32_t *__restrict__ key_new, const uint32_t *__restrict__ key_old,
+    const uint32_t *__restrict__ samp, const uint32_t *__restrict__ tsamp,
     const uint32_t *__restrict__ mov_v, const int *__restrict__ n_dev,
     const int *__restrict__ counts, uint32_t *__restrict__ mov_k, uint32_t *__restrict__ mov_q,
     uint32_t *__restrict__ mov_r, uint32_t *__restrict__ bcnt)
 {
+    __shared__ uint32_t lt[kMaxBuckets];
     const int s = *n_dev, m = counts[1];
     if (blockIdx.x * 256 >= m) return;   // whole block leaves together
+    const int nt = (s + kStayTile - 1) >> kStayShift;   // tiles (samples in tsamp)
+    const int nw = (s + 63) >> 6;                       // 64-slot samples in samp
+    for (int i = threadIdx.x; i < nt; i += 256) lt[i] = tsamp[i];
+    __syncthreads();
     // the grid covers 1/16 of the slots; more movers take further rounds
     for (int j0 = 0; j0 < m; j0 += (int)gridDim.x * 256) {
         const int j = j0 + blockIdx.x * 256 + threadIdx.x;
         const bool valid = j < m;
         const uint32_t v = valid ? mov_v[j] : 0u;
-        const uint32_t k = valid ? key_new[v] : 0u, k0 = valid ? key_old[v] : 0u;
-        const int p0 = (int)v;
-        int lo, hi;   // the insertion point lies in [lo, hi]
-        if (!valid) {
-            lo = hi = 0;
-        } else if (k0 < k) {   // (key_old[p0], p0) < (k, v): key_old[p0] != k
-            int off = 1;
-            while (p0 + off < s && kv_less(key_old[p0 + off], (uint32_t)(p0 + off), k, v)) off <<= 1;
-            lo = p0 + (off >> 1) + 1;
-            hi = min(p0 + off, s);
-        } else {
-            int off = 1;
-            while (p0 - off >= 0 && !kv_less(key_old[p0 - off], (uint32_t)(p0 - off), k, v)) off <<= 1;
-            lo = p0 - off < 0 ? 0 : p0 - off + 1;
-            hi = p0 - (off >> 1);
-        }
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (kv_less(key_old[mid], (uint32_t)mid, k, v)) lo = mid + 1; else hi = mid;
+        const uint32_t k = valid ? key_new[v] : 0u;
+        int lo = 0;
+        if (valid) {
+            // tiles whose first slot lies below (k, v)
+            int hi = nt;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (kv_less(lt[mid], (uint32_t)mid << kStayShift, k, v)) lo = mid + 1; else hi = mid;
+            }
+            if (lo > 0) {
+                // 64-slot samples below (k, v): at least the tile's first
+                const int w0 = (lo - 1) * kTileWords;
+                lo = w0 + 1;
+                hi = min(w0 + kTileWords, nw);
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (kv_less(samp[mid], (uint32_t)mid << 6, k, v)) lo = mid + 1; else hi = mid;
+                }
+                // q lies after the last such sample and at or before the next
+                const int g = (lo - 1) << 6;
+                lo = g + 1;
+                hi = min(g + 64, s);
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (kv_less(key_old[mid], (uint32_t)mid, k, v)) lo = mid + 1; else hi = mid;
+                }
+            }
         }
         // bucket slot: lanes of a wave with the same bucket share one atomic
         // (nearby movers mostly share buckets)
@@ -718,7 +737,8 @@ size_t resort_mask_words(int n_cap)
 
 void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const int *n_dev,
                         uint32_t dead_key, const uint32_t *blk_moved, const uint32_t *blk_dead,
-                        const uint64_t *mask_m, const uint64_t *mask_d, int move_block,
+                        const uint64_t *mask_m, const uint64_t *mask_d,
+                        const uint32_t *key_samp, const uint32_t *key_tsamp, int move_block,
                         const ResortScratch &r, uint32_t *out_k, uint32_t *out_v,
                         hipStream_t stream, Profiler *prof)
 {
@@ -732,7 +752,8 @@ void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const 
                 out_k, out_v, r.bcnt);
     DSMC_LAUNCH(prof, "resort_locate", 0.0, k_ins_locate, div_up(div_up(r.n_cap, 16), 256), 256,
                 stream,
-                key_new, key_old, r.mov_v, n_dev, r.counts, r.mov_k, r.mov_q, r.mov_r, r.bcnt);
+                key_new, key_old, key_samp, key_tsamp, r.mov_v, n_dev, r.counts, r.mov_k, r.mov_q,
+                r.mov_r, r.bcnt);
     DSMC_LAUNCH(prof, "resort_place", 0.0, k_ins_place, kPlaceBlocks, 256, stream, r.mov_k,
                 r.mov_v, r.mov_q, r.mov_r, n_dev, r.counts, r.bcnt, r.bk_k, r.bk_v, r.bk_q, r.bex);
     DSMC_LAUNCH(prof, "resort_merge", n * 12.0, k_ins_merge, div_up(r.n_cap, kStayTile) + 1, 256,
