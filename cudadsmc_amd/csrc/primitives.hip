@@ -374,11 +374,11 @@ __device__ __forceinline__ bool kv_less(uint32_t ka, uint32_t va, uint32_t kb, u
 // (key_old[j], j) >= (key, slot) (key_old is sorted, so this is a prefix).
 // The move kernel sampled key_old at every tile start (tsamp, 2048 slots
 // apart) and every 64 slots (samp).  The search narrows q to one tile by a
-// binary search of the tile samples in LDS, to 64 slots by the (at most 32)
-// samples of that tile, and then searches those 64 slots of key_old: at most
-// about 24 dependent steps, 13 of them in LDS and the rest within four cache
-// lines, wherever the mover lands (a move along z crosses about Nc^2 cells'
-// worth of atoms).  Also the mover's place in q's bucket.
+// binary search of the tile samples in LDS, to 64 slots by the 32 samples of
+// that tile (one 128-byte line, loaded at once), and then by a binary search
+// of those 64 slots of key_old (two lines): the same few steps wherever the
+// mover lands (a move along z crosses about Nc^2 cells' worth of atoms).
+// Also the mover's place in q's bucket.
 __global__ __launch_bounds__(256) void k_ins_locate(
     const uint32_t *__restrict__ key_new, const uint32_t *__restrict__ key_old,
     const uint32_t *__restrict__ samp, const uint32_t *__restrict__ tsamp,
@@ -391,14 +391,22 @@ __global__ __launch_bounds__(256) void k_ins_locate(
     if (blockIdx.x * 256 >= m) return;   // whole block leaves together
     const int nt = (s + kStayTile - 1) >> kStayShift;   // tiles (samples in tsamp)
     const int nw = (s + 63) >> 6;                       // 64-slot samples in samp
+    // the first round's mover is loaded while the tile samples go to LDS
+    const int jf = blockIdx.x * 256 + threadIdx.x;
+    uint32_t vn = jf < m ? mov_v[jf] : 0u;
+    uint32_t kn = jf < m ? key_new[vn] : 0u;
     for (int i = threadIdx.x; i < nt; i += 256) lt[i] = tsamp[i];
     __syncthreads();
     // the grid covers 1/16 of the slots; more movers take further rounds
     for (int j0 = 0; j0 < m; j0 += (int)gridDim.x * 256) {
         const int j = j0 + blockIdx.x * 256 + threadIdx.x;
         const bool valid = j < m;
-        const uint32_t v = valid ? mov_v[j] : 0u;
-        const uint32_t k = valid ? key_new[v] : 0u;
+        const uint32_t v = vn, k = kn;
+        if (j0 + (int)gridDim.x * 256 < m) {   // the next round's mover
+            const int jn = j + (int)gridDim.x * 256;
+            vn = jn < m ? mov_v[jn] : 0u;
+            kn = jn < m ? key_new[vn] : 0u;
+        }
         int lo = 0;
         if (valid) {
             // tiles whose first slot lies below (k, v)
@@ -408,16 +416,24 @@ __global__ __launch_bounds__(256) void k_ins_locate(
                 if (kv_less(lt[mid], (uint32_t)mid << kStayShift, k, v)) lo = mid + 1; else hi = mid;
             }
             if (lo > 0) {
-                // 64-slot samples below (k, v): at least the tile's first
+                // the tile's 32 samples (one 128-byte line, loaded at once):
+                // those below (k, v) are a prefix, at least the first
                 const int w0 = (lo - 1) * kTileWords;
-                lo = w0 + 1;
-                hi = min(w0 + kTileWords, nw);
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (kv_less(samp[mid], (uint32_t)mid << 6, k, v)) lo = mid + 1; else hi = mid;
+                const uint4 *s4 = (const uint4 *)(samp + w0);
+                uint4 sw[kTileWords / 4];
+#pragma unroll
+                for (int u = 0; u < kTileWords / 4; ++u) sw[u] = s4[u];
+                int c = 0;
+#pragma unroll
+                for (int u = 0; u < kTileWords / 4; ++u) {
+                    const int w = w0 + 4 * u;
+                    c += (w + 0 < nw && kv_less(sw[u].x, (uint32_t)(w + 0) << 6, k, v));
+                    c += (w + 1 < nw && kv_less(sw[u].y, (uint32_t)(w + 1) << 6, k, v));
+                    c += (w + 2 < nw && kv_less(sw[u].z, (uint32_t)(w + 2) << 6, k, v));
+                    c += (w + 3 < nw && kv_less(sw[u].w, (uint32_t)(w + 3) << 6, k, v));
                 }
-                // q lies after the last such sample and at or before the next
-                const int g = (lo - 1) << 6;
+                // q lies after the last such sample g and at or before g + 64
+                const int g = (w0 + c - 1) << 6;
                 lo = g + 1;
                 hi = min(g + 64, s);
                 while (lo < hi) {
