@@ -134,14 +134,18 @@ struct dsmc_ctx {
     uint32_t *srt_k[2] = {nullptr, nullptr}, *srt_v[2] = {nullptr, nullptr};  // FP32 sort outputs
     int srt_cur = 0;                // FP32: srt_*[srt_cur] holds the last sort's result
     int graph_srt = 0;              // srt_cur at graph capture
+    int graph_ssel = 0, graph_tsel = 0;   // step and trap-range buffers at graph capture
     const uint32_t *sorted_key = nullptr, *perm = nullptr;
     SortScratch sort;
     ResortScratch res;              // FP32: incremental re-sort
     double *sigvrmax = nullptr;
     dsmc_observables *obs_log = nullptr;  // (obs_cap + 1) * T records
-    long long *step_dev = nullptr;
+    // device step counter, double-buffered: the kernels of a step read
+    // step_buf[ssel] and its finalize writes the next value to the other one
+    long long *step_buf = nullptr;
+    int ssel = 0;
     long long *step_save = nullptr;
-    unsigned int *done = nullptr;
+    long long *step_dev() const { return step_buf + ssel; }
     unsigned long long *coll_accum = nullptr;
     int *n_dev = nullptr;           // [0], [1]: alive counts; [2]: scratch
     // FP64 form
@@ -160,7 +164,11 @@ struct dsmc_ctx {
     uint32_t *blk_moved = nullptr, *blk_dead = nullptr;   // per move block, for the re-sort
     uint64_t *mask_moved = nullptr, *mask_dead = nullptr; // per 64 slots, for the re-sort
     uint32_t *key_samp = nullptr, *key_tsamp = nullptr;   // samples of the last sort's keys
-    int *trap_start = nullptr;
+    // trap ranges of the slot order, double-buffered: a sorted move's
+    // finalize writes the next sort's ranges to ts_buf[tsel ^ 1]
+    int *ts_buf[2] = { nullptr, nullptr };
+    int tsel = 0;
+    int *trap_start() const { return ts_buf[tsel]; }
     double *acc = nullptr, *acc2 = nullptr;
     ObsRecords rec{};
     int max_trap_atoms = 0;
@@ -349,9 +357,8 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
     sort_scratch_bind(x->sort, N, x->alloc<char>(sort_scratch_bytes(N)));
     x->sigvrmax = x->alloc<double>(x->n_keys);
     x->obs_log = x->alloc<dsmc_observables>((size_t)(x->obs_cap + 1) * n_traps);
-    x->step_dev = x->alloc<long long>(1);
+    x->step_buf = x->alloc<long long>(2);
     x->step_save = x->alloc<long long>(1);
-    x->done = x->alloc<unsigned int>(1);
     x->coll_accum = x->alloc<unsigned long long>(n_traps);
     x->n_dev = x->alloc<int>(4);
 
@@ -397,8 +404,9 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
         x->srt_k[1] = x->alloc<uint32_t>(N); x->srt_v[1] = x->alloc<uint32_t>(N);
         x->cell_start = x->alloc<uint32_t>(x->n_keys, false);
         x->cell_end = x->alloc<uint32_t>(x->n_keys, false);
-        x->trap_start = x->alloc<int>(n_traps + 1, false);
-        DSMC_HIP(hipMemcpyAsync(x->trap_start, x->atom_begin.data(), sizeof(int) * (n_traps + 1),
+        x->ts_buf[0] = x->alloc<int>(n_traps + 1, false);
+        x->ts_buf[1] = x->alloc<int>(n_traps + 1, false);
+        DSMC_HIP(hipMemcpyAsync(x->trap_start(), x->atom_begin.data(), sizeof(int) * (n_traps + 1),
                                 hipMemcpyHostToDevice, x->stream));
         x->acc = x->alloc<double>((size_t)n_traps * kAccK);
         x->rec.stride = obs_record_stride(N);
@@ -603,7 +611,7 @@ static MoveParams move_params(const dsmc_ctx *x)
 
 static void fused_index(dsmc_ctx *x)
 {
-    launch_index_fused(x->d_traps, x->trap_start, x->T, x->fs[x->cur], x->key_a, x->n_dev + x->cur,
+    launch_index_fused(x->d_traps, x->trap_start(), x->T, x->fs[x->cur], x->key_a, x->n_dev + x->cur,
                        x->n_cap, x->stream, &x->prof);
     x->tbl_valid = false;   // the new keys decide the trap ranges of the next sort
 }
@@ -637,7 +645,7 @@ static void fused_table(dsmc_ctx *x)
     tp.n_traps = x->T;
     tp.dead_key = x->dead_key;
     launch_table_fused(x->sorted_key, x->d_key_base, tp, x->n_dev + x->cur, x->n_dev + (x->cur ^ 1),
-                       x->trap_start, x->n_cap, x->stream, &x->prof);
+                       x->trap_start(), x->n_cap, x->stream, &x->prof);
     x->tbl_valid = true;
 }
 
@@ -653,7 +661,7 @@ static void fused_collide(dsmc_ctx *x)
     fused_need_table(x);
     launch_collide_fused(x->d_traps, x->d_key_base, x->T, collide_params(x), x->sorted_key,
                          x->perm, x->n_dev + (x->cur ^ 1), x->fs[x->cur], x->sigvrmax,
-                         x->coll_accum, x->step_dev, x->n_cap, x->stream, &x->prof);
+                         x->coll_accum, x->step_dev(), x->n_cap, x->stream, &x->prof);
 }
 
 static void fused_move(dsmc_ctx *x)
@@ -669,12 +677,18 @@ static void fused_move(dsmc_ctx *x)
         mp.key_prev = x->sorted_key;
     }
     launch_move_fused(x->block, x->apt, x->d_traps, mp, x->fs[x->cur], x->fs[x->cur ^ 1],
-                      x->have_sort ? x->perm : nullptr, x->trap_start, n, x->key_a,
-                      x->evap_pos_f, x->evap_vel_f, x->rec, x->step_dev, x->n_cap, x->stream,
+                      x->have_sort ? x->perm : nullptr, x->trap_start(), n, x->key_a,
+                      x->evap_pos_f, x->evap_vel_f, x->rec, x->step_dev(), x->n_cap, x->stream,
                       &x->prof);
-    launch_obs_finalize(x->T, x->max_trap_atoms, x->block * x->apt, x->cfg.mass, x->trap_start, x->rec,
-                        x->coll_accum, x->done, x->step_dev, x->obs_log, x->obs_cap,
+    // a sorted move's finalize also leaves the next sort's trap ranges and
+    // alive count (the removed atoms go to the end)
+    launch_obs_finalize(x->T, x->max_trap_atoms, x->block * x->apt, x->cfg.mass, x->trap_start(),
+                        x->have_sort ? x->ts_buf[x->tsel ^ 1] : nullptr, x->blk_dead,
+                        x->mask_dead, x->rec, x->coll_accum, x->step_dev(),
+                        x->step_buf + (x->ssel ^ 1), x->obs_log, x->obs_cap,
                         x->have_sort ? x->n_dev + x->cur : nullptr, x->stream, &x->prof);
+    x->ssel ^= 1;
+    if (x->have_sort) x->tsel ^= 1;
     if (!x->have_sort)  // keep the count with the state it describes
         DSMC_HIP(hipMemcpyAsync(x->n_dev + (x->cur ^ 1), x->n_dev + x->cur, sizeof(int),
                                 hipMemcpyDeviceToDevice, x->stream));
@@ -700,8 +714,9 @@ static bool graph_ok(const dsmc_ctx *x) { return x->fp32 && x->cfg.use_graph && 
 
 static void fused_graph_steps(dsmc_ctx *x, int pairs)
 {
-    if (x->graph_exec && x->graph_srt != x->srt_cur) {
-        // captured with the other sort buffer holding the previous sort
+    if (x->graph_exec && (x->graph_srt != x->srt_cur || x->graph_ssel != x->ssel ||
+                          x->graph_tsel != x->tsel)) {
+        // captured with the other sort (step, trap-range) buffer current
         (void)hipGraphExecDestroy(x->graph_exec);
         (void)hipGraphDestroy(x->graph);
         x->graph_exec = nullptr;
@@ -710,6 +725,8 @@ static void fused_graph_steps(dsmc_ctx *x, int pairs)
     if (!x->graph_exec) {
         const int cur0 = x->cur;
         x->graph_srt = x->srt_cur;
+        x->graph_ssel = x->ssel;
+        x->graph_tsel = x->tsel;
         const long long step0 = x->step, cs0 = x->coll_steps;
         DSMC_HIP(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
         try {
@@ -725,6 +742,8 @@ static void fused_graph_steps(dsmc_ctx *x, int pairs)
         DSMC_HIP(hipGraphInstantiate(&x->graph_exec, x->graph, nullptr, nullptr, 0));
         x->cur = cur0;
         x->srt_cur = x->graph_srt;
+        x->ssel = x->graph_ssel;
+        x->tsel = x->graph_tsel;
         x->step = step0;
         x->coll_steps = cs0;
     }
@@ -808,7 +827,7 @@ static void observe_now(dsmc_ctx *x, std::vector<dsmc_observables> &out)
     if (x->have_sort) fail(DSMC_ESTATE, "a sort is pending: call dsmc_move_atoms first");
     DSMC_HIP(hipMemsetAsync(x->acc2, 0, sizeof(double) * kAccK * x->T, x->stream));
     hipLaunchKernelGGL(k_observe_fused, dim3(div_up(x->n_cap, 256)), dim3(256), 0, x->stream,
-                       x->d_traps, x->trap_start, x->T, x->cfg.mass, x->fs[x->cur],
+                       x->d_traps, x->trap_start(), x->T, x->cfg.mass, x->fs[x->cur],
                        x->n_dev + x->cur, x->acc2);
     DSMC_HIP(hipGetLastError());
     std::vector<double> a((size_t)x->T * kAccK);
@@ -835,11 +854,11 @@ static void reset_cells(dsmc_ctx *x)
     int nn[2] = { x->n_cap, x->n_cap };
     hipStream_t st = x->stream;
     DSMC_HIP(hipMemcpyAsync(x->sigvrmax, sv.data(), sizeof(double) * sv.size(), hipMemcpyHostToDevice, st));
-    DSMC_HIP(hipMemcpyAsync(x->step_dev, &zero, sizeof zero, hipMemcpyHostToDevice, st));
+    DSMC_HIP(hipMemcpyAsync(x->step_dev(), &zero, sizeof zero, hipMemcpyHostToDevice, st));
     DSMC_HIP(hipMemsetAsync(x->coll_accum, 0, sizeof(unsigned long long) * x->T, st));
     DSMC_HIP(hipMemcpyAsync(x->n_dev, nn, sizeof nn, hipMemcpyHostToDevice, st));
     if (x->fp32)
-        DSMC_HIP(hipMemcpyAsync(x->trap_start, x->atom_begin.data(), sizeof(int) * (x->T + 1),
+        DSMC_HIP(hipMemcpyAsync(x->trap_start(), x->atom_begin.data(), sizeof(int) * (x->T + 1),
                                 hipMemcpyHostToDevice, st));
     DSMC_HIP(hipStreamSynchronize(st));
     x->step = 0;
@@ -995,7 +1014,7 @@ static void copy_field(dsmc_ctx *x, int field, void *host, size_t bytes)
             return;
         }
         case DSMC_FIELD_TRAP_START:
-            DSMC_HIP(hipMemcpy(host, x->trap_start, need, hipMemcpyDeviceToHost)); return;
+            DSMC_HIP(hipMemcpy(host, x->trap_start(), need, hipMemcpyDeviceToHost)); return;
         default: break;
         }
     }
@@ -1105,7 +1124,7 @@ dsmc_status dsmc_init_distribution(dsmc_ctx *ctx)
         launch_init_plain(ctx->tdev[0], ip, ctx->ps[0], ctx->n_cap, ctx->stream, &ctx->prof);
         plain_index(ctx);
     } else {
-        launch_init_fused(ctx->d_traps, ctx->trap_start, ctx->T, ip, ctx->fs[0], ctx->n_cap,
+        launch_init_fused(ctx->d_traps, ctx->trap_start(), ctx->T, ip, ctx->fs[0], ctx->n_cap,
                           ctx->stream, &ctx->prof);
         fused_index(ctx);
     }
@@ -1454,7 +1473,7 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
     // compare with that sort's keys and leave the re-sort's masks, counts and
     // samples, as the step's move does; the next sort is then a full one.
     DSMC_HIP(hipStreamSynchronize(ctx->stream));
-    DSMC_HIP(hipMemcpyAsync(ctx->step_save, ctx->step_dev, sizeof(long long),
+    DSMC_HIP(hipMemcpyAsync(ctx->step_save, ctx->step_dev(), sizeof(long long),
                             hipMemcpyDeviceToDevice, ctx->stream));
     MoveParams mp = move_params(ctx);
     mp.spin_loss = 0;
@@ -1481,8 +1500,8 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
             DSMC_HIP(hipEventRecord(b, ctx->stream));
             for (int k = 0; k < 5; ++k)
                 launch_move_fused(bs, apt, ctx->d_traps, mp, ctx->fs[ctx->cur], ctx->fs[ctx->cur ^ 1],
-                                  nullptr, ctx->trap_start, ctx->n_dev + ctx->cur, ctx->res.alt_k,
-                                  ctx->evap_pos_f, ctx->evap_vel_f, ctx->rec, ctx->step_dev,
+                                  nullptr, ctx->trap_start(), ctx->n_dev + ctx->cur, ctx->res.alt_k,
+                                  ctx->evap_pos_f, ctx->evap_vel_f, ctx->rec, ctx->step_dev(),
                                   ctx->n_cap, ctx->stream, nullptr);
             DSMC_HIP(hipEventRecord(e, ctx->stream));
             DSMC_HIP(hipEventSynchronize(e));
@@ -1494,7 +1513,7 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
     ctx->prof.enabled = was;
     (void)hipEventDestroy(b);
     (void)hipEventDestroy(e);
-    DSMC_HIP(hipMemcpyAsync(ctx->step_dev, ctx->step_save, sizeof(long long),
+    DSMC_HIP(hipMemcpyAsync(ctx->step_dev(), ctx->step_save, sizeof(long long),
                             hipMemcpyDeviceToDevice, ctx->stream));
     DSMC_HIP(hipMemsetAsync(ctx->acc, 0, sizeof(double) * kAccK * ctx->T, ctx->stream));
     DSMC_HIP(hipStreamSynchronize(ctx->stream));

@@ -652,8 +652,8 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
 // Kahan-compensated velocity Verlet (moveAtoms.cu:51-91, acceleration
 // recomputed instead of stored), periodic wrap, evaporation (spin projection
 // of evaporation.cu:90-94 and/or energy cut), next cell key, and FP64
-// observables accumulated per trap.  The last block to finish writes one log
-// record per trap and advances the device step counter.
+// observables accumulated per block and trap; k_obs_finalize then writes one
+// log record per trap and advances the device step counter.
 // ---------------------------------------------------------------------------
 // One atom of the fused move, in registers between its load and its update.
 struct MoveAtom {
@@ -923,28 +923,37 @@ __global__ __launch_bounds__(BS) void k_move_fused(
 }
 
 // Grid (P, T): block (p, t) adds the records of every P-th move block of trap
-// t (fixed order), the last of the P blocks combines their results in order,
-// adds the boundary term, appends the trap's observables to the log and
-// clears the per-step accumulators; the last trap advances the step.
+// t (fixed order) and, when the next sort's trap ranges are asked for, the
+// removal counts of every P-th move block before the trap; the last of the P
+// blocks combines the partial sums in order, adds the boundary term, appends
+// the trap's observables to the log and clears the per-step accumulators.
+// The next sort puts the atoms removed now at the end, so trap t then starts
+// at its first slot minus the atoms removed before it, and the alive count is
+// the last trap's end minus all removed atoms.  With P = 1 (up to 2048 move
+// blocks per trap) no block waits for or reads what another writes.  Block
+// (0, 0) writes the next step number to the other counter.
 __global__ __launch_bounds__(256) void k_obs_finalize(
-    int T, int P, int move_block, double half_mass, int *__restrict__ trap_start,
-    const int *__restrict__ rec_trap, const double *__restrict__ rec_sum, int rec_stride,
-    double *__restrict__ lvl2, unsigned int *__restrict__ cnt, double *__restrict__ acc_over,
-    unsigned long long *__restrict__ coll_accum, unsigned int *__restrict__ done,
-    long long *__restrict__ step_dev, dsmc_observables *__restrict__ obs_log, int obs_cap,
+    int T, int P, int move_block, double half_mass, const int *__restrict__ trap_start,
+    int *__restrict__ trap_next, const uint32_t *__restrict__ blk_dead,
+    const uint64_t *__restrict__ mask_dead, const int *__restrict__ rec_trap,
+    const double *__restrict__ rec_sum, int rec_stride, double *__restrict__ lvl2,
+    unsigned int *__restrict__ cnt, double *__restrict__ acc_over,
+    unsigned long long *__restrict__ coll_accum, const long long *__restrict__ step_cur,
+    long long *__restrict__ step_next, dsmc_observables *__restrict__ obs_log, int obs_cap,
     int *__restrict__ n_next)
 {
     __shared__ double red[(256 / kWave) * kAccK];
-    __shared__ bool fin, last_trap;
-    __shared__ int tmp[256 / kWave + 1];
+    __shared__ bool fin;
     const int p = blockIdx.x, t = blockIdx.y;
-    const long long step = *step_dev;
+    const int tid = (int)threadIdx.x, stride = P * 256;
+    const long long step = *step_cur;
+    if (p == 0 && t == 0 && tid == 0) *step_next = step + 1;
     const int s0 = trap_start[t], s1 = trap_start[t + 1];
     // the per-step accumulators of the trap, loaded early (used by the block
     // that finalizes it)
     double over[kAccK];
     unsigned long long coll = 0ull;
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
 #pragma unroll
         for (int q = 0; q < kAccK; ++q) over[q] = acc_over[t * kAccK + q];
         coll = coll_accum[t];
@@ -952,9 +961,33 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
     double a[kAccK];
 #pragma unroll
     for (int q = 0; q < kAccK; ++q) a[q] = 0.0;
+    uint32_t dead = 0;   // removed atoms before s0 (this thread's share)
+    if (trap_next) {
+        // whole move blocks below s0, four counts per 16-byte load
+        const int bd = s0 / move_block, nv = bd >> 2;
+        const uint4 *v4 = (const uint4 *)blk_dead;
+        for (int v0 = p * 256 + tid; v0 < nv; v0 += 4 * stride) {
+            uint4 c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int v = v0 + u * stride;
+                c[u] = v < nv ? v4[v] : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) dead += c[u].x + c[u].y + c[u].z + c[u].w;
+        }
+        if (p == 0 && tid < (bd & 3)) dead += blk_dead[4 * nv + tid];
+        // then the 64-slot words of the block holding s0, up to s0
+        const int w0 = bd * (move_block / kWave), w1 = s0 / kWave, rem = s0 % kWave;
+        for (int w = w0 + p * 256 + tid; w < w1 + (rem != 0); w += stride) {
+            uint64_t m = mask_dead[w];
+            if (w == w1) m &= ~0ull >> (kWave - rem);
+            dead += (uint32_t)__popcll(m);
+        }
+    }
     if (s1 > s0) {
         const int bA = s0 / move_block, bB = (s1 - 1) / move_block;
-        for (int b = bA + p * 256 + (int)threadIdx.x; b <= bB; b += P * 256) {
+        for (int b = bA + p * 256 + tid; b <= bB; b += stride) {
             double r[kAccK];   // loaded together with the record's trap
 #pragma unroll
             for (int q = 0; q < kAccK; ++q) r[q] = rec_sum[(size_t)q * rec_stride + b];
@@ -964,6 +997,7 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
             }
         }
     }
+    a[kAccDead] += (double)dead;   // the records leave this sum zero
     static_assert(kAccK == 8, "eight accumulators");
     block_sum8<256>(a, red);
     // one block per trap, or the last of the trap's P blocks, finalizes it
@@ -971,8 +1005,8 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
     if (P == 1) {
 #pragma unroll
         for (int q = 0; q < kAccK; ++q) f[q] = a[q];
-        if (threadIdx.x == 0) fin = true;
-    } else if (threadIdx.x == 0) {
+        if (tid == 0) fin = true;
+    } else if (tid == 0) {
 #pragma unroll
         for (int q = 0; q < kAccK; ++q)
             __hip_atomic_store(&lvl2[((size_t)t * P + p) * kAccK + q], a[q], __ATOMIC_RELAXED,
@@ -991,60 +1025,29 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
         }
     }
     __syncthreads();
-    if (!fin) return;   // whole block leaves together
-    if (threadIdx.x == 0) {
-        for (int q = 0; q < kAccK; ++q) {
-            f[q] += over[q];
-            acc_over[t * kAccK + q] = 0.0;
-        }
-        const unsigned long long c = coll;
-        coll_accum[t] = 0ull;
-        dsmc_observables &o = obs_log[(step % obs_cap) * T + t];
-        o.ek = half_mass * (f[kAccV2x] + f[kAccV2y] + f[kAccV2z]);
-        o.ep = f[kAccEp];
-        o.v2[0] = f[kAccV2x]; o.v2[1] = f[kAccV2y]; o.v2[2] = f[kAccV2z];
-        o.e_evap = f[kAccEevap];
-        o.n_alive = (long long)(s1 - s0);
-        o.n_evap = (long long)f[kAccNevap];
-        o.collisions = (long long)c;
-        o.step = step;
-        __threadfence();
-        last_trap = atomicAdd(done, 1u) == (unsigned)(T - 1);
+    if (!fin || tid != 0) return;
+    for (int q = 0; q < kAccK; ++q) {
+        f[q] += over[q];
+        acc_over[t * kAccK + q] = 0.0;
     }
-    __syncthreads();
-    if (!last_trap) return;
-    __threadfence();
-    // Last trap finalized: every log record of this step is written.  When
-    // the move followed a sort, the next sort puts the atoms removed now at
-    // the end, so trap t's range shrinks by its removed atoms: the next
-    // step's trap ranges and alive count follow without scanning the keys.
-    if (n_next) {
-        dsmc_observables *row = obs_log + (step % obs_cap) * T;
-        auto left = [&](int tt) {   // atoms of trap tt that stay alive
-            const long long na = __hip_atomic_load(&row[tt].n_alive, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-            const long long ne = __hip_atomic_load(&row[tt].n_evap, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-            return (int)(na - ne);
-        };
-        const int per = (T + 255) / 256;
-        const int t0 = min(T, (int)threadIdx.x * per), t1 = min(T, t0 + per);
-        int sum = 0;
-        for (int tt = t0; tt < t1; ++tt) sum += left(tt);
-        int total;
-        int run = block_exclusive_scan<256>(sum, tmp, &total);
-        for (int tt = t0; tt < t1; ++tt) {
-            trap_start[tt] = run;
-            run += left(tt);
+    coll_accum[t] = 0ull;
+    dsmc_observables &o = obs_log[(step % obs_cap) * T + t];
+    o.ek = half_mass * (f[kAccV2x] + f[kAccV2y] + f[kAccV2z]);
+    o.ep = f[kAccEp];
+    o.v2[0] = f[kAccV2x]; o.v2[1] = f[kAccV2y]; o.v2[2] = f[kAccV2z];
+    o.e_evap = f[kAccEevap];
+    o.n_alive = (long long)(s1 - s0);
+    o.n_evap = (long long)f[kAccNevap];
+    o.collisions = (long long)coll;
+    o.step = step;
+    if (trap_next) {
+        const int d0 = (int)f[kAccDead];
+        trap_next[t] = s0 - d0;
+        if (t == T - 1) {
+            const int nn = s1 - d0 - (int)f[kAccNevap];
+            trap_next[T] = nn;
+            *n_next = nn;
         }
-        if (threadIdx.x == 0) {
-            trap_start[T] = total;
-            *n_next = total;
-        }
-    }
-    if (threadIdx.x == 0) {
-        *done = 0u;
-        *step_dev = step + 1;
     }
 }
 
@@ -1189,17 +1192,18 @@ void launch_move_fused(int block, int apt, const TrapDev *traps, const MoveParam
 }
 
 void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, double mass,
-                         int *trap_start,
-                         const ObsRecords &rec, unsigned long long *coll_accum, unsigned int *done,
-                         long long *step_dev, dsmc_observables *obs_log, int obs_cap, int *n_next,
-                         hipStream_t st, Profiler *prof)
+                         const int *trap_start, int *trap_next, const uint32_t *blk_dead,
+                         const uint64_t *mask_dead, const ObsRecords &rec,
+                         unsigned long long *coll_accum, const long long *step_cur,
+                         long long *step_next, dsmc_observables *obs_log, int obs_cap,
+                         int *n_next, hipStream_t st, Profiler *prof)
 {
     const int blocks = div_up(max_trap_atoms > 0 ? max_trap_atoms : 1, move_block) + 1;
     const int P = std::min(kMaxFinalizeP, div_up(blocks, 2048));
     DSMC_LAUNCH(prof, "obs_finalize", (double)n_traps * 64.0, k_obs_finalize, dim3(P, n_traps), 256,
-                st, n_traps, P, move_block, 0.5 * mass, trap_start, (const int *)rec.trap,
-                (const double *)rec.sum, rec.stride, rec.lvl2, rec.cnt, rec.acc_over, coll_accum,
-                done, step_dev, obs_log, obs_cap, n_next);
+                st, n_traps, P, move_block, 0.5 * mass, trap_start, trap_next, blk_dead, mask_dead,
+                (const int *)rec.trap, (const double *)rec.sum, rec.stride, rec.lvl2, rec.cnt,
+                rec.acc_over, coll_accum, step_cur, step_next, obs_log, obs_cap, n_next);
 }
 
 void launch_gather_fused(const FusedPtrs &dst, const FusedPtrs &src, const uint32_t *map,

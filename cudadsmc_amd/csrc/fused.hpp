@@ -69,6 +69,9 @@ __device__ __forceinline__ void rec_store(f4 *r, uint32_t j, const FState &a)
 
 // Observable accumulators per trap (doubles).
 enum { kAccEk = 0, kAccEp, kAccV2x, kAccV2y, kAccV2z, kAccEevap, kAccNevap, kAccN, kAccK };
+// the move's records leave kAccN zero; the finalize sums there the atoms
+// removed in the slots before the trap
+constexpr int kAccDead = kAccN;
 
 struct MoveParams {
     double dt;
@@ -143,11 +146,15 @@ void launch_move_fused(int block, int apt, const TrapDev *traps, const MoveParam
 // When n_next is set (the move followed a sort), the last block also writes
 // the next step's trap ranges (trap_start) and alive count (*n_next): each
 // trap loses its atoms removed in this step, which the next sort puts last.
+// trap_next / n_next (when not null): the trap ranges and alive count after
+// the next sort, from the move's per-block removal counts and masks.  step
+// is read from step_cur; step + 1 is written to step_next.
 void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, double mass,
-                         int *trap_start,
-                         const ObsRecords &rec, unsigned long long *coll_accum, unsigned int *done,
-                         long long *step_dev, dsmc_observables *obs_log, int obs_cap, int *n_next,
-                         hipStream_t st, Profiler *prof);
+                         const int *trap_start, int *trap_next, const uint32_t *blk_dead,
+                         const uint64_t *mask_dead, const ObsRecords &rec,
+                         unsigned long long *coll_accum, const long long *step_cur,
+                         long long *step_next, dsmc_observables *obs_log, int obs_cap,
+                         int *n_next, hipStream_t st, Profiler *prof);
 void launch_gather_fused(const FusedPtrs &dst, const FusedPtrs &src, const uint32_t *map,
                          const int *n_dev, int n_cap, int spin, hipStream_t st, Profiler *prof);
 double move_fused_bytes_per_atom(int spin);
