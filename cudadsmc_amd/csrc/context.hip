@@ -135,6 +135,7 @@ struct dsmc_ctx {
     int srt_cur = 0;                // FP32: srt_*[srt_cur] holds the last sort's result
     int graph_srt = 0;              // srt_cur at graph capture
     int graph_ssel = 0, graph_tsel = 0;   // step and trap-range buffers at graph capture
+    bool prepped = false;           // FP32: the last finalize ran the re-sort's first phase
     const uint32_t *sorted_key = nullptr, *perm = nullptr;
     SortScratch sort;
     ResortScratch res;              // FP32: incremental re-sort
@@ -627,11 +628,12 @@ static void fused_sort(dsmc_ctx *x)
     if (x->srt_valid && x->cfg.sort_mode == DSMC_SORT_INCREMENTAL && resort_supported(x->n_cap))
         resort_incremental(x->key_a, x->srt_k[x->srt_cur], n, x->dead_key, x->blk_moved,
                            x->blk_dead, x->mask_moved, x->mask_dead, x->key_samp, x->key_tsamp,
-                           x->block * x->apt, x->res,
+                           x->block * x->apt, x->prepped, x->res,
                            ok, ov, x->stream, &x->prof);
     else
         radix_sort(x->key_a, nullptr, x->res.alt_k, x->res.alt_v, ok, ov, n, x->key_bits,
                    x->sort, x->stream, &x->prof);
+    x->prepped = false;
     x->srt_cur ^= 1;
     x->sorted_key = ok;
     x->perm = ov;
@@ -681,12 +683,19 @@ static void fused_move(dsmc_ctx *x)
                       x->evap_pos_f, x->evap_vel_f, x->rec, x->step_dev(), x->n_cap, x->stream,
                       &x->prof);
     // a sorted move's finalize also leaves the next sort's trap ranges and
-    // alive count (the removed atoms go to the end)
+    // alive count (the removed atoms go to the end), and runs the first phase
+    // of the incremental re-sort that follows
+    ResortPrepArgs prep;
+    x->prepped = x->have_sort && x->cfg.sort_mode == DSMC_SORT_INCREMENTAL &&
+                 resort_supported(x->n_cap);
+    if (x->prepped)
+        prep = resort_prep_args(x->res, n, x->block * x->apt, x->blk_moved, x->blk_dead,
+                                x->mask_moved, x->mask_dead);
     launch_obs_finalize(x->T, x->max_trap_atoms, x->block * x->apt, x->cfg.mass, x->trap_start(),
                         x->have_sort ? x->ts_buf[x->tsel ^ 1] : nullptr, x->blk_dead,
                         x->mask_dead, x->rec, x->coll_accum, x->step_dev(),
                         x->step_buf + (x->ssel ^ 1), x->obs_log, x->obs_cap,
-                        x->have_sort ? x->n_dev + x->cur : nullptr, x->stream, &x->prof);
+                        x->have_sort ? x->n_dev + x->cur : nullptr, prep, x->stream, &x->prof);
     x->ssel ^= 1;
     if (x->have_sort) x->tsel ^= 1;
     if (!x->have_sort)  // keep the count with the state it describes

@@ -103,6 +103,22 @@ struct ResortScratch {
     uint32_t *tile_cnt = nullptr, *tile_dead = nullptr, *bcnt = nullptr, *bex = nullptr;
     int *counts = nullptr;       // [0] stays, [1] movers, [2] and [3] removed
 };
+// Arguments of the re-sort's first phase when another kernel runs it (the
+// fused finalize); blocks = 0: not run there.
+struct ResortPrepArgs {
+    int blocks = 0;
+    int per_tile = 0;              // move blocks per tile
+    const int *n_dev = nullptr;    // slots the move wrote
+    const uint32_t *blk_moved = nullptr, *blk_dead = nullptr;
+    const uint64_t *mask_m = nullptr, *mask_d = nullptr;
+    uint32_t *pre_m = nullptr, *pre_d = nullptr, *mov_v = nullptr, *bcnt = nullptr;
+    int *counts = nullptr;
+};
+// The first phase of an incremental re-sort of n_dev slots that follows a
+// move of move_block-slot blocks.
+ResortPrepArgs resort_prep_args(const ResortScratch &r, const int *n_dev, int move_block,
+                                const uint32_t *blk_moved, const uint32_t *blk_dead,
+                                const uint64_t *mask_m, const uint64_t *mask_d);
 size_t resort_scratch_bytes(int n_cap);
 void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem);
 bool resort_supported(int n_cap);
@@ -112,12 +128,14 @@ size_t resort_mask_words(int n_cap);
 // changed (not removed) and the removed atoms; mask_m / mask_d: the same
 // atoms as one bit per slot; key_samp / key_tsamp: key_old[64 w] for every
 // 64-slot word w and key_old[kResortTile t] for every tile t below n (all
-// written by the fused move).
+// written by the fused move).  prepped: the first phase (resort_prep_block)
+// already ran on these masks (in the fused finalize).
 void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const int *n_dev,
                         uint32_t dead_key, const uint32_t *blk_moved, const uint32_t *blk_dead,
                         const uint64_t *mask_m, const uint64_t *mask_d, const uint32_t *key_samp,
-                        const uint32_t *key_tsamp, int move_block, const ResortScratch &r,
-                        uint32_t *out_k, uint32_t *out_v, hipStream_t stream, Profiler *prof);
+                        const uint32_t *key_tsamp, int move_block, bool prepped,
+                        const ResortScratch &r, uint32_t *out_k, uint32_t *out_v,
+                        hipStream_t stream, Profiler *prof);
 
 // Exclusive scan of n ints (n known on the host); partial holds div_up(n, tile).
 void exclusive_scan(const int *in, int *out, int n, int *partial, hipStream_t stream,

@@ -20,6 +20,7 @@
 // Every kernel reads the alive count and the step number from device memory,
 // so the loop can be captured once and replayed as a HIP graph.
 #include "fused.hpp"
+#include "resort_device.hpp"
 
 #include <cstdlib>
 
@@ -931,7 +932,8 @@ __global__ __launch_bounds__(BS) void k_move_fused(
 // at its first slot minus the atoms removed before it, and the alive count is
 // the last trap's end minus all removed atoms.  With P = 1 (up to 2048 move
 // blocks per trap) no block waits for or reads what another writes.  Block
-// (0, 0) writes the next step number to the other counter.
+// (0, 0) writes the next step number to the other counter.  Blocks from P T
+// on run the next incremental re-sort's first phase (resort_prep_block).
 __global__ __launch_bounds__(256) void k_obs_finalize(
     int T, int P, int move_block, double half_mass, const int *__restrict__ trap_start,
     int *__restrict__ trap_next, const uint32_t *__restrict__ blk_dead,
@@ -940,11 +942,19 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
     unsigned int *__restrict__ cnt, double *__restrict__ acc_over,
     unsigned long long *__restrict__ coll_accum, const long long *__restrict__ step_cur,
     long long *__restrict__ step_next, dsmc_observables *__restrict__ obs_log, int obs_cap,
-    int *__restrict__ n_next)
+    int *__restrict__ n_next, ResortPrepArgs prep)
 {
+    static_assert(kPrepWords == 256, "prep blocks are finalize blocks");
     __shared__ double red[(256 / kWave) * kAccK];
+    __shared__ uint32_t ptmp[256 / kWave + 1];
     __shared__ bool fin;
-    const int p = blockIdx.x, t = blockIdx.y;
+    if ((int)blockIdx.x >= P * T) {
+        resort_prep_block((int)blockIdx.x - P * T, *prep.n_dev, prep.blk_moved, prep.blk_dead,
+                          prep.per_tile, prep.mask_m, prep.mask_d, prep.pre_m, prep.pre_d,
+                          prep.counts, prep.mov_v, prep.bcnt, ptmp);
+        return;
+    }
+    const int p = (int)blockIdx.x % P, t = (int)blockIdx.x / P;
     const int tid = (int)threadIdx.x, stride = P * 256;
     const long long step = *step_cur;
     if (p == 0 && t == 0 && tid == 0) *step_next = step + 1;
@@ -1196,14 +1206,16 @@ void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, double
                          const uint64_t *mask_dead, const ObsRecords &rec,
                          unsigned long long *coll_accum, const long long *step_cur,
                          long long *step_next, dsmc_observables *obs_log, int obs_cap,
-                         int *n_next, hipStream_t st, Profiler *prof)
+                         int *n_next, const ResortPrepArgs &prep, hipStream_t st,
+                         Profiler *prof)
 {
     const int blocks = div_up(max_trap_atoms > 0 ? max_trap_atoms : 1, move_block) + 1;
     const int P = std::min(kMaxFinalizeP, div_up(blocks, 2048));
-    DSMC_LAUNCH(prof, "obs_finalize", (double)n_traps * 64.0, k_obs_finalize, dim3(P, n_traps), 256,
-                st, n_traps, P, move_block, 0.5 * mass, trap_start, trap_next, blk_dead, mask_dead,
-                (const int *)rec.trap, (const double *)rec.sum, rec.stride, rec.lvl2, rec.cnt,
-                rec.acc_over, coll_accum, step_cur, step_next, obs_log, obs_cap, n_next);
+    DSMC_LAUNCH(prof, "obs_finalize", (double)n_traps * 64.0, k_obs_finalize,
+                P * n_traps + prep.blocks, 256, st, n_traps, P, move_block, 0.5 * mass,
+                trap_start, trap_next, blk_dead, mask_dead, (const int *)rec.trap,
+                (const double *)rec.sum, rec.stride, rec.lvl2, rec.cnt, rec.acc_over, coll_accum,
+                step_cur, step_next, obs_log, obs_cap, n_next, prep);
 }
 
 void launch_gather_fused(const FusedPtrs &dst, const FusedPtrs &src, const uint32_t *map,

@@ -149,12 +149,15 @@ void launch_move_fused(int block, int apt, const TrapDev *traps, const MoveParam
 // trap_next / n_next (when not null): the trap ranges and alive count after
 // the next sort, from the move's per-block removal counts and masks.  step
 // is read from step_cur; step + 1 is written to step_next.
+// prep.blocks > 0: the same launch also runs the next incremental re-sort's
+// first phase on the move's masks.
 void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, double mass,
                          const int *trap_start, int *trap_next, const uint32_t *blk_dead,
                          const uint64_t *mask_dead, const ObsRecords &rec,
                          unsigned long long *coll_accum, const long long *step_cur,
                          long long *step_next, dsmc_observables *obs_log, int obs_cap,
-                         int *n_next, hipStream_t st, Profiler *prof);
+                         int *n_next, const ResortPrepArgs &prep, hipStream_t st,
+                         Profiler *prof);
 void launch_gather_fused(const FusedPtrs &dst, const FusedPtrs &src, const uint32_t *map,
                          const int *n_dev, int n_cap, int spin, hipStream_t st, Profiler *prof);
 double move_fused_bytes_per_atom(int spin);

@@ -14,6 +14,7 @@
 // None of these kernels waits on another block, so none can hang.
 #include "dsmc_device.hpp"
 #include "dsmc_internal.hpp"
+#include "resort_device.hpp"
 
 namespace dsmc {
 
@@ -206,10 +207,10 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 // crossed a cell face change key (a few percent); the others ("stays") keep
 // their order, so nothing is sorted but the movers, and the stays are never
 // copied out of slot order:
-//   1. tiles: movers and removed atoms before every 2048-slot tile
-//      (prefix sums of the move kernel's per-block counts);
-//   2. collect: the movers' (key, slot) in slot order; removed atoms are
-//      written straight to the end of the output, in slot order;
+//   1-2. prep: movers and removed atoms before every 2048-slot tile (prefix
+//      sums of the move kernel's per-block counts) and the movers' slots in
+//      slot order (resort_prep_block; normally run by the fused finalize
+//      right after the move, alongside its per-trap sums);
 //   3. locate: each mover's insertion point q in slot space, the first slot
 //      j with (key_old[j], j) >= (key, slot) (binary search of key_old, whose
 //      stays equal key_new, through samples of it that the move kernel
@@ -221,147 +222,25 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 //      their final positions: with S = stays before the tile plus movers in
 //      earlier buckets, a stay goes to S + (stays of the tile before it) +
 //      (bucket movers with q <= its slot), a mover of rank r to S + (stays
-//      of the tile below q) + r.
+//      of the tile below q) + r.  The tile's removed atoms go to the end of
+//      the output, in slot order.
 // No kernel waits on another block.
 // ---------------------------------------------------------------------------
-constexpr int kStayTile = kResortTile;         // slots per tile (and bucket)
-constexpr int kStayShift = 11;
-constexpr int kTileWords = kStayTile / 64;     // mask words per tile
 constexpr int kBucketLds = 1024;               // movers a bucket ranks in LDS
 constexpr int kBucketBits = 13;
 constexpr int kMaxBuckets = 1 << kBucketBits;  // LDS bucket table of the place kernel
-static_assert((1 << kStayShift) == kStayTile, "stay tile");
-static_assert(kTileWords == 32, "a tile's mask words are half a wave");
 
-// Per tile: movers and removed atoms before it (exclusive prefixes of the
-// counts the move kernel left per move block); entry `tiles` holds the
-// totals, and counts[] = {stays, movers, removed}.  Block k handles tiles
-// [256 k, 256 k + 256) and adds up the earlier tiles itself (no block waits
-// for another).
-constexpr int kTilesBlock = 256;
-
-__global__ __launch_bounds__(kTilesBlock) void k_ins_tiles(const uint32_t *__restrict__ blk_moved,
-                                                           const uint32_t *__restrict__ blk_dead,
-                                                           int per_tile, const int *__restrict__ n_dev,
-                                                           uint32_t *__restrict__ pre_m,
-                                                           uint32_t *__restrict__ pre_d,
-                                                           int *__restrict__ counts)
-{
-    __shared__ uint32_t tmp[kTilesBlock / kWave + 1];
-    const int n = *n_dev;
-    const int tiles = n > 0 ? (n - 1) / kStayTile + 1 : 1;
-    const int t0 = blockIdx.x * kTilesBlock;
-    if (t0 >= tiles) return;   // whole block leaves together
-    // this thread's tile: per_tile (2 .. 32) entries, loaded before the prefix
-    const int t = t0 + (int)threadIdx.x;
-    uint32_t sm = 0, sd = 0;
-    if (t < tiles) {
-        if (per_tile >= 4) {
-            const uint4 *vm = (const uint4 *)(blk_moved + (size_t)t * per_tile);
-            const uint4 *vd = (const uint4 *)(blk_dead + (size_t)t * per_tile);
-            uint4 a[8], d[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                a[q] = q < per_tile / 4 ? vm[q] : make_uint4(0, 0, 0, 0);
-                d[q] = q < per_tile / 4 ? vd[q] : make_uint4(0, 0, 0, 0);
-            }
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                sm += a[q].x + a[q].y + a[q].z + a[q].w;
-                sd += d[q].x + d[q].y + d[q].z + d[q].w;
-            }
-        } else {
-            for (int b = t * per_tile; b < (t + 1) * per_tile; ++b) { sm += blk_moved[b]; sd += blk_dead[b]; }
-        }
-    }
-    // earlier tiles: their move blocks [0, t0 * per_tile), a multiple of 1024
-    // entries read as 16-byte vectors, four per thread in flight
-    uint32_t bm = 0, bd = 0;
-    {
-        const uint4 *vm = (const uint4 *)blk_moved, *vd = (const uint4 *)blk_dead;
-        const int nv = t0 * per_tile / 4;
-        for (int v0 = 0; v0 < nv; v0 += 4 * kTilesBlock) {
-            uint4 a[4], d[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int v = v0 + u * kTilesBlock + (int)threadIdx.x;
-                a[u] = v < nv ? vm[v] : make_uint4(0, 0, 0, 0);
-                d[u] = v < nv ? vd[v] : make_uint4(0, 0, 0, 0);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                bm += a[u].x + a[u].y + a[u].z + a[u].w;
-                bd += d[u].x + d[u].y + d[u].z + d[u].w;
-            }
-        }
-    }
-    uint32_t tot;
-    (void)block_exclusive_scan<kTilesBlock>(bm, tmp, &tot);
-    bm = tot;
-    (void)block_exclusive_scan<kTilesBlock>(bd, tmp, &tot);
-    bd = tot;
-    uint32_t tm, td;
-    const uint32_t em = block_exclusive_scan<kTilesBlock>(sm, tmp, &tm);
-    const uint32_t ed = block_exclusive_scan<kTilesBlock>(sd, tmp, &td);
-    if (t < tiles) {
-        pre_m[t] = bm + em;
-        pre_d[t] = bd + ed;
-    }
-    if (t0 + kTilesBlock >= tiles && threadIdx.x == 0) {
-        const uint32_t M = bm + tm, D = bd + td;
-        pre_m[tiles] = M;
-        pre_d[tiles] = D;
-        counts[0] = n - (int)M - (int)D;
-        counts[1] = (int)M;
-        counts[2] = (int)D;
-    }
-}
-
-// Exclusive prefix of v over the 32 lanes of this lane's half-wave.
-__device__ __forceinline__ uint32_t halfwave_exclusive_scan(uint32_t v)
-{
-    const int hl = threadIdx.x & 31;
-    uint32_t s = v;
-#pragma unroll
-    for (int d = 1; d < 32; d <<= 1) {
-        const uint32_t u = __shfl_up(s, d, 32);
-        if (hl >= d) s += u;
-    }
-    return s - v;
-}
-
-// One thread per 64-slot mask word (a tile's 32 words are half a wave):
-// writes the slots of the word's movers, in slot order, to mov_v and its
-// removed atoms to the end of the output.  Also clears the bucket counts.
-__global__ __launch_bounds__(256) void k_ins_collect(
+// Stand-alone first phase (when the fused finalize did not run it): see
+// resort_prep_block.
+__global__ __launch_bounds__(kPrepWords) void k_ins_prep(
+    const uint32_t *__restrict__ blk_moved, const uint32_t *__restrict__ blk_dead, int per_tile,
     const uint64_t *__restrict__ mask_m, const uint64_t *__restrict__ mask_d,
-    const int *__restrict__ n_dev, uint32_t dead_key, const uint32_t *__restrict__ pre_m,
-    const uint32_t *__restrict__ pre_d, const int *__restrict__ counts,
-    uint32_t *__restrict__ mov_v, uint32_t *__restrict__ out_k, uint32_t *__restrict__ out_v,
-    uint32_t *__restrict__ bcnt)
+    const int *__restrict__ n_dev, uint32_t *__restrict__ pre_m, uint32_t *__restrict__ pre_d,
+    int *__restrict__ counts, uint32_t *__restrict__ mov_v, uint32_t *__restrict__ bcnt)
 {
-    const int n = *n_dev;
-    const int g = blockIdx.x * 256 + (int)threadIdx.x;
-    if (g <= (n >> kStayShift)) bcnt[g] = 0u;   // buckets 0 .. n / 2048
-    const int words = (n + 63) >> 6;
-    if ((int)blockIdx.x * 256 >= words) return;   // whole block leaves together
-    const bool valid = g < words;
-    const int tile = g >> 5;
-    uint64_t mm = 0ull, md = 0ull;
-    uint32_t om = 0u, od = 0u;
-    if (valid) {
-        mm = mask_m[g]; md = mask_d[g];
-        om = pre_m[tile];
-        od = (uint32_t)(n - counts[2]) + pre_d[tile];
-    }
-    om += halfwave_exclusive_scan((uint32_t)__popcll(mm));
-    od += halfwave_exclusive_scan((uint32_t)__popcll(md));
-    for (uint64_t b = mm; b; b &= b - 1)
-        mov_v[om++] = (uint32_t)g * 64u + (uint32_t)(__ffsll((unsigned long long)b) - 1);
-    for (uint64_t b = md; b; b &= b - 1) {
-        out_k[od] = dead_key;
-        out_v[od++] = (uint32_t)g * 64u + (uint32_t)(__ffsll((unsigned long long)b) - 1);
-    }
+    __shared__ uint32_t tmp[kPrepWords / kWave + 1];
+    resort_prep_block((int)blockIdx.x, *n_dev, blk_moved, blk_dead, per_tile, mask_m, mask_d, pre_m,
+                      pre_d, counts, mov_v, bcnt, tmp);
 }
 
 // (key, slot) order; slots are distinct.
@@ -602,7 +481,8 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     const uint32_t *__restrict__ pre_m, const uint32_t *__restrict__ pre_d,
     const uint32_t *__restrict__ bk_k, const uint32_t *__restrict__ bk_v,
     const uint32_t *__restrict__ bk_q, const uint32_t *__restrict__ bcnt,
-    const uint32_t *__restrict__ bex, uint32_t *__restrict__ srt_k, uint32_t *__restrict__ srt_v,
+    const uint32_t *__restrict__ bex, const int *__restrict__ counts, uint32_t dead_key,
+    uint32_t *__restrict__ srt_k, uint32_t *__restrict__ srt_v,
     uint32_t *__restrict__ srt_q, uint32_t *__restrict__ alt_k, uint32_t *__restrict__ alt_v,
     uint32_t *__restrict__ alt_q, uint32_t *__restrict__ out_k, uint32_t *__restrict__ out_v)
 {
@@ -628,14 +508,24 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     }
     if (tid < kTileWords) {
         const int l0 = tid * 64;
-        uint64_t st = 0ull;
+        uint64_t st = 0ull, dd = 0ull;
+        uint32_t od = 0u;
         if (l0 < ns) {
             const int w = (g0 >> 6) + tid;
             const uint64_t in = ns - l0 >= 64 ? ~0ull : (1ull << (ns - l0)) - 1ull;
-            st = ~(mask_m[w] | mask_d[w]) & in;
+            const uint64_t md = mask_d[w];
+            st = ~(mask_m[w] | md) & in;
+            dd = md & in;
+            od = (uint32_t)(n - counts[2]) + pre_d[t];
         }
         stw[tid] = st;
         wst[tid] = halfwave_exclusive_scan((uint32_t)__popcll(st));
+        // removed atoms: to the end of the output, in slot order
+        od += halfwave_exclusive_scan((uint32_t)__popcll(dd));
+        for (uint64_t bb = dd; bb; bb &= bb - 1) {
+            out_k[od] = dead_key;
+            out_v[od++] = (uint32_t)(g0 + l0) + (uint32_t)(__ffsll((unsigned long long)bb) - 1);
+        }
     }
 #pragma unroll
     for (int u = 0; u < kMergePer; ++u) H[tid + 256 * u] = 0u;
@@ -751,21 +641,33 @@ size_t resort_mask_words(int n_cap)
     return (size_t)div_up(n_cap > 0 ? n_cap : 1, kStayTile) * kTileWords + kTileWords;
 }
 
+ResortPrepArgs resort_prep_args(const ResortScratch &r, const int *n_dev, int move_block,
+                                const uint32_t *blk_moved, const uint32_t *blk_dead,
+                                const uint64_t *mask_m, const uint64_t *mask_d)
+{
+    ResortPrepArgs a;
+    a.blocks = resort_prep_blocks(r.n_cap);
+    a.per_tile = kStayTile / move_block;
+    a.n_dev = n_dev;
+    a.blk_moved = blk_moved; a.blk_dead = blk_dead;
+    a.mask_m = mask_m; a.mask_d = mask_d;
+    a.pre_m = r.tile_cnt; a.pre_d = r.tile_dead; a.mov_v = r.mov_v; a.bcnt = r.bcnt;
+    a.counts = r.counts;
+    return a;
+}
+
 void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const int *n_dev,
                         uint32_t dead_key, const uint32_t *blk_moved, const uint32_t *blk_dead,
                         const uint64_t *mask_m, const uint64_t *mask_d,
                         const uint32_t *key_samp, const uint32_t *key_tsamp, int move_block,
-                        const ResortScratch &r, uint32_t *out_k, uint32_t *out_v,
+                        bool prepped, const ResortScratch &r, uint32_t *out_k, uint32_t *out_v,
                         hipStream_t stream, Profiler *prof)
 {
     const double n = (double)r.n_cap;
-    const int words = div_up(r.n_cap, 64) + 2;
-    DSMC_LAUNCH(prof, "resort_tiles", 0.0, k_ins_tiles, div_up(r.tiles, kTilesBlock), kTilesBlock,
-                stream, blk_moved, blk_dead, kStayTile / move_block, n_dev, r.tile_cnt,
-                r.tile_dead, r.counts);
-    DSMC_LAUNCH(prof, "resort_collect", 0.0, k_ins_collect, div_up(words, 256), 256, stream,
-                mask_m, mask_d, n_dev, dead_key, r.tile_cnt, r.tile_dead, r.counts, r.mov_v,
-                out_k, out_v, r.bcnt);
+    if (!prepped)
+        DSMC_LAUNCH(prof, "resort_prep", 0.0, k_ins_prep, resort_prep_blocks(r.n_cap), kPrepWords,
+                    stream, blk_moved, blk_dead, kStayTile / move_block, mask_m, mask_d, n_dev,
+                    r.tile_cnt, r.tile_dead, r.counts, r.mov_v, r.bcnt);
     DSMC_LAUNCH(prof, "resort_locate", 0.0, k_ins_locate, div_up(div_up(r.n_cap, 16), 256), 256,
                 stream,
                 key_new, key_old, key_samp, key_tsamp, r.mov_v, n_dev, r.counts, r.mov_k, r.mov_q,
@@ -774,8 +676,8 @@ void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const 
                 r.mov_v, r.mov_q, r.mov_r, n_dev, r.counts, r.bcnt, r.bk_k, r.bk_v, r.bk_q, r.bex);
     DSMC_LAUNCH(prof, "resort_merge", n * 12.0, k_ins_merge, div_up(r.n_cap, kStayTile) + 1, 256,
                 stream, key_new, mask_m, mask_d, n_dev, r.tile_cnt, r.tile_dead, r.bk_k, r.bk_v,
-                r.bk_q, r.bcnt, r.bex, r.srt_k, r.srt_v, r.srt_q, r.alt_k, r.alt_v, r.spare, out_k,
-                out_v);
+                r.bk_q, r.bcnt, r.bex, r.counts, dead_key, r.srt_k, r.srt_v, r.srt_q, r.alt_k,
+                r.alt_v, r.spare, out_k, out_v);
 }
 
 // ---------------------------------------------------------------------------
