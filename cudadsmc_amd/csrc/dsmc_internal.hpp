@@ -97,11 +97,12 @@ constexpr int kResortTile = 2048;
 struct ResortScratch {
     int n_cap = 0, tiles = 0;
     uint32_t *mov_k = nullptr, *mov_v = nullptr;
-    uint32_t *mov_q = nullptr, *mov_r = nullptr, *bk_k = nullptr, *bk_v = nullptr, *bk_q = nullptr;
+    uint32_t *mov_q = nullptr, *mov_r = nullptr;
+    uint32_t *bk_k = nullptr, *bk_v = nullptr, *bk_q = nullptr;   // buckets' fixed rooms
     uint32_t *srt_k = nullptr, *srt_v = nullptr, *srt_q = nullptr;
     uint32_t *alt_k = nullptr, *alt_v = nullptr, *spare = nullptr;
-    uint32_t *tile_cnt = nullptr, *tile_dead = nullptr, *bcnt = nullptr, *bex = nullptr;
-    int *counts = nullptr;       // [0] stays, [1] movers, [2] and [3] removed
+    uint32_t *tile_cnt = nullptr, *tile_dead = nullptr, *bcnt = nullptr;
+    int *counts = nullptr;       // [0] stays, [1] movers, [2] removed, [3] bucket overflow
 };
 // Arguments of the re-sort's first phase when another kernel runs it (the
 // fused finalize); blocks = 0: not run there.

@@ -227,6 +227,7 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 // No kernel waits on another block.
 // ---------------------------------------------------------------------------
 constexpr int kBucketLds = 1024;               // movers a bucket ranks in LDS
+constexpr int kBucketCap = kBucketLds;         // fixed room of a bucket
 constexpr int kBucketBits = 13;
 constexpr int kMaxBuckets = 1 << kBucketBits;  // LDS bucket table of the place kernel
 
@@ -257,13 +258,16 @@ __device__ __forceinline__ bool kv_less(uint32_t ka, uint32_t va, uint32_t kb, u
 // that tile (one 128-byte line, loaded at once), and then by a binary search
 // of those 64 slots of key_old (two lines): the same few steps wherever the
 // mover lands (a move along z crosses about Nc^2 cells' worth of atoms).
-// Also the mover's place in q's bucket.
+// Each mover then takes the next place of q's bucket (the 2048-slot tile of
+// q): a bucket has a fixed room of kBucketCap movers, and the movers beyond
+// it go to an overflow list.
 __global__ __launch_bounds__(256) void k_ins_locate(
     const uint32_t *__restrict__ key_new, const uint32_t *__restrict__ key_old,
     const uint32_t *__restrict__ samp, const uint32_t *__restrict__ tsamp,
-    const uint32_t *__restrict__ mov_v, const int *__restrict__ n_dev,
-    const int *__restrict__ counts, uint32_t *__restrict__ mov_k, uint32_t *__restrict__ mov_q,
-    uint32_t *__restrict__ mov_r, uint32_t *__restrict__ bcnt)
+    const uint32_t *__restrict__ mov_v, const int *__restrict__ n_dev, int *__restrict__ counts,
+    uint32_t *__restrict__ fb_k, uint32_t *__restrict__ fb_v, uint32_t *__restrict__ fb_q,
+    uint32_t *__restrict__ ov_k, uint32_t *__restrict__ ov_v, uint32_t *__restrict__ ov_q,
+    uint32_t *__restrict__ bcnt)
 {
     __shared__ uint32_t lt[kMaxBuckets];
     const int s = *n_dev, m = counts[1];
@@ -331,67 +335,15 @@ __global__ __launch_bounds__(256) void k_ins_locate(
             base = atomicAdd(&bcnt[bkt], (uint32_t)__popcll(peers));
         base = __shfl(base, leader < 0 ? 0 : leader, kWave);
         if (valid) {
-            mov_k[j] = k;
-            mov_q[j] = (uint32_t)lo;
-            mov_r[j] = base + (uint32_t)__popcll(peers & lanemask_lt());
+            const uint32_t r = base + (uint32_t)__popcll(peers & lanemask_lt());
+            if (r < (uint32_t)kBucketCap) {   // the bucket's fixed room
+                const size_t o = (size_t)bkt * kBucketCap + r;
+                fb_k[o] = k; fb_v[o] = v; fb_q[o] = (uint32_t)lo;
+            } else {   // beyond it: the overflow list
+                const int o = atomicAdd(&counts[kOverflowCount], 1);
+                ov_k[o] = k; ov_v[o] = v; ov_q[o] = (uint32_t)lo;
+            }
         }
-    }
-}
-
-// Exclusive scan of the bucket counts into LDS (every block computes it).
-__device__ __forceinline__ void bucket_scan(const uint32_t *__restrict__ bcnt, int nbk, uint32_t *ex,
-                                            uint32_t *tmp)
-{
-    const int per = (nbk + 255) / 256;
-    const int b0 = threadIdx.x * per;
-    uint32_t s = 0;
-    for (int i = 0; i < per; ++i) if (b0 + i < nbk) s += bcnt[b0 + i];
-    uint32_t tot;
-    uint32_t run = block_exclusive_scan<256>(s, tmp, &tot);
-    for (int i = 0; i < per; ++i) if (b0 + i < nbk) { ex[b0 + i] = run; run += bcnt[b0 + i]; }
-    __syncthreads();
-}
-
-constexpr int kPlaceBlocks = 256;
-
-__global__ __launch_bounds__(256) void k_ins_place(
-    const uint32_t *__restrict__ mov_k, const uint32_t *__restrict__ mov_v,
-    const uint32_t *__restrict__ mov_q, const uint32_t *__restrict__ mov_r,
-    const int *__restrict__ n_dev, const int *__restrict__ counts,
-    const uint32_t *__restrict__ bcnt,
-    uint32_t *__restrict__ bk_k, uint32_t *__restrict__ bk_v, uint32_t *__restrict__ bk_q,
-    uint32_t *__restrict__ bex)
-{
-    __shared__ uint32_t ex[kMaxBuckets];
-    __shared__ uint32_t tmp[256 / kWave + 1];
-    const int m = counts[1];
-    const int nbk = (*n_dev >> kStayShift) + 1;
-    // this thread's first kPU movers are loaded before the bucket scan
-    constexpr int kPU = 4;
-    const int j0 = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
-    uint32_t ck[kPU], cv[kPU], cq[kPU], cr[kPU];
-#pragma unroll
-    for (int u = 0; u < kPU; ++u) {
-        const int j = j0 + u * stride;
-        const bool in = j < m;
-        ck[u] = in ? mov_k[j] : 0u; cv[u] = in ? mov_v[j] : 0u;
-        cq[u] = in ? mov_q[j] : 0u; cr[u] = in ? mov_r[j] : 0u;
-    }
-    bucket_scan(bcnt, nbk, ex, tmp);
-    if (blockIdx.x == 0)   // bucket offsets for the merge
-        for (int i = threadIdx.x; i < nbk; i += 256) bex[i] = ex[i];
-#pragma unroll
-    for (int u = 0; u < kPU; ++u) {
-        if (j0 + u * stride >= m) break;
-        const uint32_t o = ex[cq[u] >> kStayShift] + cr[u];
-        bk_k[o] = ck[u]; bk_v[o] = cv[u]; bk_q[o] = cq[u];
-    }
-    for (int j = j0 + kPU * stride; j < m; j += stride) {
-        const uint32_t q = mov_q[j];
-        const uint32_t o = ex[q >> kStayShift] + mov_r[j];
-        bk_k[o] = mov_k[j];
-        bk_v[o] = mov_v[j];
-        bk_q[o] = q;
     }
 }
 
@@ -479,9 +431,10 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     const uint32_t *__restrict__ key_new, const uint64_t *__restrict__ mask_m,
     const uint64_t *__restrict__ mask_d, const int *__restrict__ n_dev,
     const uint32_t *__restrict__ pre_m, const uint32_t *__restrict__ pre_d,
-    const uint32_t *__restrict__ bk_k, const uint32_t *__restrict__ bk_v,
-    const uint32_t *__restrict__ bk_q, const uint32_t *__restrict__ bcnt,
-    const uint32_t *__restrict__ bex, const int *__restrict__ counts, uint32_t dead_key,
+    const uint32_t *__restrict__ fb_k, const uint32_t *__restrict__ fb_v,
+    const uint32_t *__restrict__ fb_q, const uint32_t *__restrict__ ov_k,
+    const uint32_t *__restrict__ ov_v, const uint32_t *__restrict__ ov_q,
+    const uint32_t *__restrict__ bcnt, const int *__restrict__ counts, uint32_t dead_key,
     uint32_t *__restrict__ srt_k, uint32_t *__restrict__ srt_v,
     uint32_t *__restrict__ srt_q, uint32_t *__restrict__ alt_k, uint32_t *__restrict__ alt_v,
     uint32_t *__restrict__ alt_q, uint32_t *__restrict__ out_k, uint32_t *__restrict__ out_v)
@@ -491,6 +444,7 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     __shared__ uint64_t stw[kTileWords];   // stay bits of the tile's words
     __shared__ uint32_t wst[kTileWords];   // stays of the tile before each word
     __shared__ uint32_t tmp[256 / kWave + 1];
+    __shared__ int n_big;   // movers gathered for a big bucket
     const int n = *n_dev;
     const int t = blockIdx.x;
     if (t > (n >> kStayShift)) return;   // whole block leaves together
@@ -506,6 +460,20 @@ __global__ __launch_bounds__(256) void k_ins_merge(
         const int p = tid + 256 * u;
         kk[u] = p < ns ? key_new[g0 + p] : 0u;
     }
+    // the tile's counts, and the bucket counts of earlier tiles, in flight
+    // with the keys
+    const uint32_t pm = pre_m[t], pd = pre_d[t];
+    const int b = (int)bcnt[t];
+    uint32_t bsum = 0;
+    {
+        const uint4 *c4 = (const uint4 *)bcnt;
+        const int nv = t >> 2;
+        for (int v = tid; v < nv; v += 256) {
+            const uint4 c = c4[v];
+            bsum += c.x + c.y + c.z + c.w;
+        }
+        if (tid < (t & 3)) bsum += bcnt[4 * nv + tid];
+    }
     if (tid < kTileWords) {
         const int l0 = tid * 64;
         uint64_t st = 0ull, dd = 0ull;
@@ -516,7 +484,7 @@ __global__ __launch_bounds__(256) void k_ins_merge(
             const uint64_t md = mask_d[w];
             st = ~(mask_m[w] | md) & in;
             dd = md & in;
-            od = (uint32_t)(n - counts[2]) + pre_d[t];
+            od = (uint32_t)(n - counts[2]) + pd;
         }
         stw[tid] = st;
         wst[tid] = halfwave_exclusive_scan((uint32_t)__popcll(st));
@@ -529,25 +497,38 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     }
 #pragma unroll
     for (int u = 0; u < kMergePer; ++u) H[tid + 256 * u] = 0u;
-    const uint32_t before = bex[t];   // movers in earlier buckets
-    const int b = (int)bcnt[t];
+    // movers in earlier buckets
+    uint32_t before;
+    (void)block_exclusive_scan<256>(bsum, tmp, &before);   // syncs: H cleared, stay words in place
     // first output slot of the tile: stays of earlier tiles + earlier movers
-    const uint32_t w0 = (uint32_t)g0 - pre_m[t] - pre_d[t] + before;
-    const uint32_t *bk = bk_k + before, *bv = bk_v + before, *bq = bk_q + before;
-    const bool small = b <= kBucketLds;
+    const uint32_t w0 = (uint32_t)g0 - pm - pd + before;
+    const size_t fb0 = (size_t)t * kBucketCap;   // the bucket's fixed room
+    const bool small = b <= kBucketCap;
     const uint32_t *gk = nullptr, *gv = nullptr, *gq = nullptr;   // sorted big bucket
-    __syncthreads();   // H cleared, stay words in place
     if (small) {
         for (int e = tid; e < b; e += 256) {
-            const uint32_t q = bq[e];
-            lk[e] = bk[e]; lv[e] = bv[e]; lq[e] = q;
+            const uint32_t q = fb_q[fb0 + e];
+            lk[e] = fb_k[fb0 + e]; lv[e] = fb_v[fb0 + e]; lq[e] = q;
             atomicAdd(&H[q - (uint32_t)g0], 1u);
         }
     } else {
-        // copy the bucket to scratch and sort it there
+        // gather the bucket (its room, then its movers in the overflow list)
+        // in scratch and sort it there
         uint32_t *ak = srt_k + before, *av = srt_v + before, *aq = srt_q + before;
         uint32_t *ck = alt_k + before, *cv = alt_v + before, *cq = alt_q + before;
-        for (int e = tid; e < b; e += 256) { ak[e] = bk[e]; av[e] = bv[e]; aq[e] = bq[e]; }
+        for (int e = tid; e < kBucketCap; e += 256) {
+            ak[e] = fb_k[fb0 + e]; av[e] = fb_v[fb0 + e]; aq[e] = fb_q[fb0 + e];
+        }
+        if (tid == 0) n_big = kBucketCap;
+        __syncthreads();
+        const int no = counts[kOverflowCount];
+        for (int e = tid; e < no; e += 256) {
+            const uint32_t q = ov_q[e];
+            if ((q >> kStayShift) == (uint32_t)t) {
+                const int o = atomicAdd(&n_big, 1);
+                ak[o] = ov_k[e]; av[o] = ov_v[e]; aq[o] = q;
+            }
+        }
         __syncthreads();
         const int which = big_bucket_sort(ak, av, aq, ck, cv, cq, b, lk, lv, lq);
         gk = which ? ck : ak; gv = which ? cv : av; gq = which ? cq : aq;
@@ -604,11 +585,17 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     }
 }
 
+// Per-slot arrays (movers, big-bucket scratch, radix alternates), the
+// buckets' fixed rooms (kBucketCap movers for each of the tiles + 1 buckets),
+// and per-tile counts.
+static size_t resort_room(size_t tiles) { return (tiles + 1) * (size_t)kBucketCap; }
+
 size_t resort_scratch_bytes(int n_cap)
 {
     const size_t n = (size_t)(n_cap > 0 ? n_cap : 1);
-    const size_t tiles = (size_t)div_up((long long)n, kStayTile) + 2;
-    return 13 * n * sizeof(uint32_t) + 4 * tiles * sizeof(uint32_t) + 8 * sizeof(int) + 24 * 256;
+    const size_t tiles = (size_t)div_up((long long)n, kStayTile);
+    return 10 * n * sizeof(uint32_t) + 3 * resort_room(tiles) * sizeof(uint32_t) +
+           4 * (tiles + 2) * sizeof(uint32_t) + 8 * sizeof(int) + 24 * 256;
 }
 
 void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem)
@@ -618,13 +605,14 @@ void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem)
     r.n_cap = (int)n;
     r.tiles = div_up((long long)n, kStayTile);
     char *p = align((char *)mem);
-    uint32_t **arrays[13] = {&r.mov_k, &r.mov_v, &r.mov_q, &r.mov_r, &r.bk_k,  &r.bk_v, &r.bk_q,
-                             &r.srt_k, &r.srt_v, &r.srt_q, &r.alt_k, &r.alt_v, &r.spare};
+    uint32_t **arrays[10] = {&r.mov_k, &r.mov_v, &r.mov_q, &r.mov_r, &r.srt_k,
+                             &r.srt_v, &r.srt_q, &r.alt_k, &r.alt_v, &r.spare};
     for (auto a : arrays) { *a = (uint32_t *)p; p = align(p + n * sizeof(uint32_t)); }
+    uint32_t **rooms[3] = {&r.bk_k, &r.bk_v, &r.bk_q};
+    for (auto a : rooms) { *a = (uint32_t *)p; p = align(p + resort_room(r.tiles) * sizeof(uint32_t)); }
     r.tile_cnt = (uint32_t *)p; p = align(p + (size_t)(r.tiles + 2) * sizeof(uint32_t));
     r.tile_dead = (uint32_t *)p; p = align(p + (size_t)(r.tiles + 2) * sizeof(uint32_t));
     r.bcnt = (uint32_t *)p; p = align(p + (size_t)(r.tiles + 2) * sizeof(uint32_t));
-    r.bex = (uint32_t *)p; p = align(p + (size_t)(r.tiles + 2) * sizeof(uint32_t));
     r.counts = (int *)p;
 }
 
@@ -668,16 +656,14 @@ void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const 
         DSMC_LAUNCH(prof, "resort_prep", 0.0, k_ins_prep, resort_prep_blocks(r.n_cap), kPrepWords,
                     stream, blk_moved, blk_dead, kStayTile / move_block, mask_m, mask_d, n_dev,
                     r.tile_cnt, r.tile_dead, r.counts, r.mov_v, r.bcnt);
+    // the overflow list reuses the mover arrays that the locate does not read
     DSMC_LAUNCH(prof, "resort_locate", 0.0, k_ins_locate, div_up(div_up(r.n_cap, 16), 256), 256,
-                stream,
-                key_new, key_old, key_samp, key_tsamp, r.mov_v, n_dev, r.counts, r.mov_k, r.mov_q,
-                r.mov_r, r.bcnt);
-    DSMC_LAUNCH(prof, "resort_place", 0.0, k_ins_place, kPlaceBlocks, 256, stream, r.mov_k,
-                r.mov_v, r.mov_q, r.mov_r, n_dev, r.counts, r.bcnt, r.bk_k, r.bk_v, r.bk_q, r.bex);
+                stream, key_new, key_old, key_samp, key_tsamp, r.mov_v, n_dev, r.counts, r.bk_k,
+                r.bk_v, r.bk_q, r.mov_k, r.mov_r, r.mov_q, r.bcnt);
     DSMC_LAUNCH(prof, "resort_merge", n * 12.0, k_ins_merge, div_up(r.n_cap, kStayTile) + 1, 256,
                 stream, key_new, mask_m, mask_d, n_dev, r.tile_cnt, r.tile_dead, r.bk_k, r.bk_v,
-                r.bk_q, r.bcnt, r.bex, r.counts, dead_key, r.srt_k, r.srt_v, r.srt_q, r.alt_k,
-                r.alt_v, r.spare, out_k, out_v);
+                r.bk_q, r.mov_k, r.mov_r, r.mov_q, r.bcnt, r.counts, dead_key, r.srt_k, r.srt_v,
+                r.srt_q, r.alt_k, r.alt_v, r.spare, out_k, out_v);
 }
 
 // ---------------------------------------------------------------------------

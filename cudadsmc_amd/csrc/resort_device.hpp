@@ -12,6 +12,7 @@ constexpr int kStayTile = kResortTile;         // slots per tile (and bucket)
 constexpr int kStayShift = 11;
 constexpr int kTileWords = kStayTile / 64;     // mask words per tile
 constexpr int kPrepWords = 256;                // mask words per prep block (8 tiles)
+constexpr int kOverflowCount = 3;              // counts[]: movers in the buckets' overflow list
 static_assert((1 << kStayShift) == kStayTile, "stay tile");
 static_assert(kTileWords == 32, "a tile's mask words are half a wave");
 static_assert(kPrepWords % kTileWords == 0, "prep blocks hold whole tiles");
@@ -42,7 +43,8 @@ __host__ __device__ inline int resort_prep_blocks(int n_cap)
 //     move's per-block counts below it (every block adds them up itself, so
 //     no block waits for another), those before each word a block scan;
 //   * pre_m / pre_d: movers and removed atoms before each tile, entry
-//     `tiles` the totals; counts = {stays, movers, removed} (last block);
+//     `tiles` the totals; counts = {stays, movers, removed, 0 overflow}
+//     (last block);
 //   * mov_v: the movers' slots in slot order;
 //   * the bucket counts are cleared.
 // per_tile = move blocks per tile (kStayTile / move block size).
@@ -101,6 +103,7 @@ __device__ __forceinline__ void resort_prep_block(
         counts[0] = n - (int)M - (int)D;
         counts[1] = (int)M;
         counts[2] = (int)D;
+        counts[kOverflowCount] = 0;
     }
     uint32_t o = om;
     for (uint64_t b = mm; b; b &= b - 1)
