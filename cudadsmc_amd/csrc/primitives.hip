@@ -497,16 +497,23 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     }
 #pragma unroll
     for (int u = 0; u < kMergePer; ++u) H[tid + 256 * u] = 0u;
+    const size_t fb0 = (size_t)t * kBucketCap;   // the bucket's fixed room
+    const bool small = b <= kBucketCap;
+    // this thread's first mover of a small bucket, loaded before the scan
+    uint32_t mk = 0u, mv = 0u, mq = 0u;
+    if (small && tid < b) { mk = fb_k[fb0 + tid]; mv = fb_v[fb0 + tid]; mq = fb_q[fb0 + tid]; }
     // movers in earlier buckets
     uint32_t before;
     (void)block_exclusive_scan<256>(bsum, tmp, &before);   // syncs: H cleared, stay words in place
     // first output slot of the tile: stays of earlier tiles + earlier movers
     const uint32_t w0 = (uint32_t)g0 - pm - pd + before;
-    const size_t fb0 = (size_t)t * kBucketCap;   // the bucket's fixed room
-    const bool small = b <= kBucketCap;
     const uint32_t *gk = nullptr, *gv = nullptr, *gq = nullptr;   // sorted big bucket
     if (small) {
-        for (int e = tid; e < b; e += 256) {
+        if (tid < b) {
+            lk[tid] = mk; lv[tid] = mv; lq[tid] = mq;
+            atomicAdd(&H[mq - (uint32_t)g0], 1u);
+        }
+        for (int e = tid + 256; e < b; e += 256) {
             const uint32_t q = fb_q[fb0 + e];
             lk[e] = fb_k[fb0 + e]; lv[e] = fb_v[fb0 + e]; lq[e] = q;
             atomicAdd(&H[q - (uint32_t)g0], 1u);
