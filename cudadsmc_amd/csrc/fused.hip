@@ -509,32 +509,41 @@ __device__ __forceinline__ int run_cell(const CollideParams &cp, const CellJob &
 __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     const TrapDev *__restrict__ traps, const int *__restrict__ key_base, int T, CollideParams cp,
     const uint32_t *__restrict__ key, const uint32_t *__restrict__ perm,
-    const int *__restrict__ n_dev, FusedPtrs s, double *__restrict__ sigvrmax,
+    const int *__restrict__ n_dev, int n_cap, FusedPtrs s, double *__restrict__ sigvrmax,
     unsigned long long *__restrict__ coll_accum, const long long *__restrict__ step_dev,
     int wave_trials)
 {
     constexpr int kChunk = kWave * kCollItems;
     extern __shared__ int kb[];
     __shared__ int heads[kFBlock / kWave][kChunk + 1];
+    __shared__ uint32_t hkey[kFBlock / kWave][kChunk];   // key of each head
     __shared__ short multi[kFBlock / kWave][kChunk];   // heads index of cells with >= 2 atoms
     __shared__ short heavy[kFBlock / kWave][kChunk];
-    const int n = *n_dev;
-    const uint32_t step = (uint32_t)*step_dev;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int chunk0 = (blockIdx.x * (kFBlock / kWave) + w) * kChunk;
-    const int chunk1 = min(n, chunk0 + kChunk);
     const uint64_t lt = lanemask_lt();
-    // the chunk's keys and the two beyond its ends (kNoKey: none), loaded
-    // before the trap table so that the loads overlap
+    // the chunk's keys, the one before it and the 64 after it (for the end
+    // of its last cell) are loaded with the alive count, the step and the
+    // trap table; kNoKey marks slots past the alive atoms
     constexpr uint32_t kNoKey = 0xFFFFFFFFu;   // keys are below 2^31
     uint32_t kk[kCollItems];
 #pragma unroll
     for (int it = 0; it < kCollItems; ++it) {
         const int i = chunk0 + it * kWave + lane;
-        kk[it] = i < n ? key[i] : kNoKey;
+        kk[it] = i < n_cap ? key[i] : kNoKey;
     }
-    const uint32_t k_before = chunk0 > 0 && chunk0 <= n ? key[chunk0 - 1] : kNoKey;
-    const uint32_t k_after = chunk0 + kChunk < n ? key[chunk0 + kChunk] : kNoKey;
+    const int jp = chunk0 + kChunk + lane;
+    uint32_t pk = jp < n_cap ? key[jp] : kNoKey;
+    uint32_t k_before = chunk0 > 0 && chunk0 <= n_cap ? key[chunk0 - 1] : kNoKey;
+    const int n = *n_dev;
+    const uint32_t step = (uint32_t)*step_dev;
+    const int chunk1 = min(n, chunk0 + kChunk);
+#pragma unroll
+    for (int it = 0; it < kCollItems; ++it)
+        if (chunk0 + it * kWave + lane >= n) kk[it] = kNoKey;
+    if (jp >= n) pk = kNoKey;
+    if (chunk0 > n) k_before = kNoKey;
+    const uint32_t k_after = __shfl(pk, 0, kWave);
     load_table(kb, key_base, T + 1);
     int nh = 0, nm = 0;
     uint32_t klast = 0;   // key of the chunk's last cell head
@@ -556,7 +565,7 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
         }
         const uint64_t bh = __ballot(head), bm = __ballot(mult);
         const int pos = nh + __popcll(bh & lt);
-        if (head) heads[w][pos] = i;
+        if (head) { heads[w][pos] = i; hkey[w][pos] = k; }
         if (mult) multi[w][nm + __popcll(bm & lt)] = (short)pos;
         if (bh) klast = __shfl(k, 63 - __clzll((long long)bh), kWave);
         nh += __popcll(bh);
@@ -566,8 +575,7 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
         // end of the last cell: the first later slot with another key.  The
         // wave probes the next 64 slots at once; a longer cell (e.g. the
         // overflow cell of atoms outside the grid) is searched by galloping.
-        const int j = chunk1 + lane;
-        const uint64_t bd = __ballot(j < n && key[j] != klast);
+        const uint64_t bd = __ballot(chunk1 == chunk0 + kChunk && pk != kNoKey && pk != klast);
         int end = chunk1 + kWave;
         if (bd) end = chunk1 + __ffsll((unsigned long long)bd) - 1;
         if (chunk1 + kWave >= n) end = min(end, n);
@@ -588,7 +596,7 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     // traps of the chunk's first and last cells: usually the same
     int t_lo = 0, t_hi = 0;
     if (nh > 0) {
-        t_lo = __builtin_amdgcn_readfirstlane(last_le(kb, T + 1, (int)key[chunk0]));
+        t_lo = __builtin_amdgcn_readfirstlane(last_le(kb, T + 1, (int)__shfl(kk[0], 0, kWave)));
         t_hi = __builtin_amdgcn_readfirstlane(last_le(kb, T + 1, (int)klast));
     }
     __syncthreads();
@@ -604,7 +612,7 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
         if (mi < nm) {
             h = multi[w][mi];
             const int i = heads[w][h], end = heads[w][h + 1];
-            const uint32_t k = key[i];
+            const uint32_t k = hkey[w][h];
             auto setup = [&](const TrapDev &tr, int t) {
                 stream = tr.stream;
                 trial0_load(cp, stream, k - (uint32_t)tr.key_base, step, end - i, i, perm, s, t0);
@@ -631,7 +639,7 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     for (int q = 0; q < nheavy; ++q) {
         const int h = multi[w][heavy[w][q]];
         const int i = heads[w][h];
-        const uint32_t k = key[i];
+        const uint32_t k = hkey[w][h];
         const int t = t_lo == t_hi ? t_lo : last_le(kb, T + 1, (int)k);
         CellJob c;
         (void)cell_setup(traps[t], t, k, cp, sigvrmax, i, heads[w][h + 1], c);
@@ -1143,8 +1151,8 @@ void launch_collide_fused(const TrapDev *traps, const int *key_base, int n_traps
     const int wave_trials = env ? std::atoi(env) : kWaveTrialsDefault;
     if (prof && prof->enabled) prof->begin(st, "collide_fused", (double)n_cap * 4.0);
     hipLaunchKernelGGL(k_collide_fused, dim3(fgrid(n_cap, kFBlock * kCollItems)), dim3(kFBlock), lds,
-                       st, traps, key_base, n_traps, cp, key, perm, n_dev, s, sigvrmax, coll_accum,
-                       step_dev, wave_trials);
+                       st, traps, key_base, n_traps, cp, key, perm, n_dev, n_cap, s, sigvrmax,
+                       coll_accum, step_dev, wave_trials);
     DSMC_HIP(hipGetLastError());
     if (prof && prof->enabled) prof->end(st);
 }
