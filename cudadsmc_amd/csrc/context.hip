@@ -890,12 +890,12 @@ static InitParams init_params(const dsmc_ctx *x)
 }
 
 // (plane, component) of each axis' value and remainder in the FP32 state
-// records: position x, y, z = (0,0)+(0,3), (0,1)+(1,0), (0,2)+(1,1); velocity
-// x, y, z = (1,2)+(2,1), (1,3)+(2,2), (2,0)+(2,3).
+// records: position x, y, z = (0,0)+(0,3), (0,1)+(2,0), (0,2)+(2,1); velocity
+// x, y, z = (1,0)+(1,3), (1,1)+(2,2), (1,2)+(2,3).
 static const int kRecPosHi[3][2] = { { 0, 0 }, { 0, 1 }, { 0, 2 } };
-static const int kRecPosLo[3][2] = { { 0, 3 }, { 1, 0 }, { 1, 1 } };
-static const int kRecVelHi[3][2] = { { 1, 2 }, { 1, 3 }, { 2, 0 } };
-static const int kRecVelLo[3][2] = { { 2, 1 }, { 2, 2 }, { 2, 3 } };
+static const int kRecPosLo[3][2] = { { 0, 3 }, { 2, 0 }, { 2, 1 } };
+static const int kRecVelHi[3][2] = { { 1, 0 }, { 1, 1 }, { 1, 2 } };
+static const int kRecVelLo[3][2] = { { 1, 3 }, { 2, 2 }, { 2, 3 } };
 
 // ---------------------------------------------------------------------------
 // Host-side field conversion (device layout -> AoS host)

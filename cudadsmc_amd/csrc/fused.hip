@@ -122,10 +122,10 @@ __global__ __launch_bounds__(kFBlock) void k_index_fused(const TrapDev *__restri
     const int t = last_le(ts, T, i);
     const TrapDev &tr = traps[t];
     const f4 *q = s.r + rec_index((uint32_t)i);
-    const f4 p0 = q[0], p1 = q[kRecTile];
+    const f4 p0 = q[0], p2 = q[2 * kRecTile];
     const double X = (double)p0.x + (double)p0.w;
-    const double Y = (double)p0.y + (double)p1.x;
-    const double Z = (double)p0.z + (double)p1.y;
+    const double Y = (double)p0.y + (double)p2.x;
+    const double Z = (double)p0.z + (double)p2.y;
     key[i] = (uint32_t)(tr.key_base + cell_of_fast(tr, X, Y, Z));
 }
 
@@ -305,17 +305,19 @@ __device__ __forceinline__ void trial_draw(const CollideParams &cp, uint32_t str
 
 typedef float f2 __attribute__((ext_vector_type(2)));
 
-// (vx, vy) of slot j: upper half of plane 1; vz: first float of plane 2.
-__device__ __forceinline__ f2 *vel_xy(f4 *r, uint32_t j) { return (f2 *)(r + rec_index(j) + kRecTile) + 1; }
-__device__ __forceinline__ f4 *vel_zr(f4 *r, uint32_t j) { return r + rec_index(j) + 2 * kRecTile; }
+// (vx, vy, vz, rvx) of slot j: plane 1; (rvy, rvz): upper half of plane 2.
+__device__ __forceinline__ f4 *vel_rec(f4 *r, uint32_t j) { return r + rec_index(j) + kRecTile; }
+__device__ __forceinline__ f2 *vel_rem_yz(f4 *r, uint32_t j)
+{
+    return (f2 *)(r + rec_index(j) + 2 * kRecTile) + 1;
+}
 
 __device__ __forceinline__ float rel_speed(const FusedPtrs &s, uint32_t a, uint32_t b)
 {
-    const f2 ha = *vel_xy(s.r, a), hb = *vel_xy(s.r, b);
-    const float za = ((const float *)vel_zr(s.r, a))[0], zb = ((const float *)vel_zr(s.r, b))[0];
-    const float dx = ha.x - hb.x;
-    const float dy = ha.y - hb.y;
-    const float dz = za - zb;
+    const f4 va = *vel_rec(s.r, a), vb = *vel_rec(s.r, b);
+    const float dx = va.x - vb.x;
+    const float dy = va.y - vb.y;
+    const float dz = va.z - vb.z;
     return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
@@ -356,16 +358,16 @@ __device__ __forceinline__ void scatter_pair(const CollideParams &cp, uint32_t s
                                              uint32_t step, int tr, const FusedPtrs &s, uint32_t a,
                                              uint32_t b)
 {
-    f2 *const ha = vel_xy(s.r, a), *const hb = vel_xy(s.r, b);
-    f4 *const za = vel_zr(s.r, a), *const zb = vel_zr(s.r, b);
-    const f2 va = *ha, vb = *hb;
-    const f4 wa = *za, wb = *zb;   // (vz, rvx, rvy, rvz)
-    const double vax = (double)va.x + (double)wa.y;
-    const double vay = (double)va.y + (double)wa.z;
-    const double vaz = (double)wa.x + (double)wa.w;
-    const double vbx = (double)vb.x + (double)wb.y;
-    const double vby = (double)vb.y + (double)wb.z;
-    const double vbz = (double)wb.x + (double)wb.w;
+    f4 *const ha = vel_rec(s.r, a), *const hb = vel_rec(s.r, b);
+    f2 *const za = vel_rem_yz(s.r, a), *const zb = vel_rem_yz(s.r, b);
+    const f4 va = *ha, vb = *hb;   // (vx, vy, vz, rvx)
+    const f2 wa = *za, wb = *zb;   // (rvy, rvz)
+    const double vax = (double)va.x + (double)va.w;
+    const double vay = (double)va.y + (double)wa.x;
+    const double vaz = (double)va.z + (double)wa.y;
+    const double vbx = (double)vb.x + (double)vb.w;
+    const double vby = (double)vb.y + (double)wb.x;
+    const double vbz = (double)vb.z + (double)wb.y;
     const double ddx = vax - vbx, ddy = vay - vby, ddz = vaz - vbz;
     const double magVrel = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
     const double cx = 0.5 * (vax + vbx), cy = 0.5 * (vay + vby), cz = 0.5 * (vaz + vbz);
@@ -379,10 +381,10 @@ __device__ __forceinline__ void scatter_pair(const CollideParams &cp, uint32_t s
     split_df(cx + 0.5 * wx, nb.vx, nb.rvx);
     split_df(cy + 0.5 * wy, nb.vy, nb.rvy);
     split_df(cz + 0.5 * wz, nb.vz, nb.rvz);
-    *ha = f2{ na.vx, na.vy };
-    *za = f4{ na.vz, na.rvx, na.rvy, na.rvz };
-    *hb = f2{ nb.vx, nb.vy };
-    *zb = f4{ nb.vz, nb.rvx, nb.rvy, nb.rvz };
+    *ha = f4{ na.vx, na.vy, na.vz, na.rvx };
+    *za = f2{ na.rvy, na.rvz };
+    *hb = f4{ nb.vx, nb.vy, nb.vz, nb.rvx };
+    *zb = f2{ nb.rvy, nb.rvz };
 }
 
 // The first trial of a cell (its pair, acceptance uniform and relative

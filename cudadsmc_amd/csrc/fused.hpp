@@ -14,17 +14,20 @@ namespace dsmc {
 // that the previous updates could not represent (position p + rp, velocity
 // v + rv).
 struct FState {
-    float px, py, pz, rpx, rpy, rpz;
-    float vx, vy, vz, rvx, rvy, rvz;
+    // in the order of the record planes below, so that each plane is one
+    // 16-byte piece of the struct
+    float px, py, pz, rpx;
+    float vx, vy, vz, rvx;
+    float rpy, rpz, rvy, rvz;
 };
 
 // The twelve floats of every slot are stored in tiles of 64 slots; a tile is
 // three planes of 64 four-float records,
-//   plane 0: (px, py, pz, rpx)   plane 1: (rpy, rpz, vx, vy)
-//   plane 2: (vz, rvx, rvy, rvz)
+//   plane 0: (px, py, pz, rpx)   plane 1: (vx, vy, vz, rvx)
+//   plane 2: (rpy, rpz, rvy, rvz)
 // so a slot's state is three 16-byte accesses, a wave's access to a plane is
-// 1 KiB contiguous, and the velocity with its remainder (what a collision
-// reads and writes) is the upper half of plane 1 and plane 2.
+// 1 KiB contiguous, and the velocity (what a collision trial reads) is one
+// 16-byte record; an accepted collision also writes the upper half of plane 2.
 typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int kRecTile = 64;
 constexpr int kRecPlanes = 3;
@@ -46,8 +49,8 @@ __device__ __forceinline__ void rec_load(const f4 *r, uint32_t j, FState &a)
     const f4 *q = r + rec_index(j);
     const f4 p0 = q[0], p1 = q[kRecTile], p2 = q[2 * kRecTile];
     a.px = p0.x; a.py = p0.y; a.pz = p0.z; a.rpx = p0.w;
-    a.rpy = p1.x; a.rpz = p1.y; a.vx = p1.z; a.vy = p1.w;
-    a.vz = p2.x; a.rvx = p2.y; a.rvy = p2.z; a.rvz = p2.w;
+    a.vx = p1.x; a.vy = p1.y; a.vz = p1.z; a.rvx = p1.w;
+    a.rpy = p2.x; a.rpz = p2.y; a.rvy = p2.z; a.rvz = p2.w;
 }
 
 // Non-temporal: for streaming writes of the whole state.
@@ -55,16 +58,16 @@ __device__ __forceinline__ void rec_store_stream(f4 *r, uint32_t j, const FState
 {
     f4 *q = r + rec_index(j);
     __builtin_nontemporal_store(f4{ a.px, a.py, a.pz, a.rpx }, q);
-    __builtin_nontemporal_store(f4{ a.rpy, a.rpz, a.vx, a.vy }, q + kRecTile);
-    __builtin_nontemporal_store(f4{ a.vz, a.rvx, a.rvy, a.rvz }, q + 2 * kRecTile);
+    __builtin_nontemporal_store(f4{ a.vx, a.vy, a.vz, a.rvx }, q + kRecTile);
+    __builtin_nontemporal_store(f4{ a.rpy, a.rpz, a.rvy, a.rvz }, q + 2 * kRecTile);
 }
 
 __device__ __forceinline__ void rec_store(f4 *r, uint32_t j, const FState &a)
 {
     f4 *q = r + rec_index(j);
     q[0] = f4{ a.px, a.py, a.pz, a.rpx };
-    q[kRecTile] = f4{ a.rpy, a.rpz, a.vx, a.vy };
-    q[2 * kRecTile] = f4{ a.vz, a.rvx, a.rvy, a.rvz };
+    q[kRecTile] = f4{ a.vx, a.vy, a.vz, a.rvx };
+    q[2 * kRecTile] = f4{ a.rpy, a.rpz, a.rvy, a.rvz };
 }
 
 // Observable accumulators per trap (doubles).
