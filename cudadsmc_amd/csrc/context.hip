@@ -136,6 +136,7 @@ struct dsmc_ctx {
     int graph_srt = 0;              // srt_cur at graph capture
     int graph_ssel = 0, graph_tsel = 0;   // step and trap-range buffers at graph capture
     bool prepped = false;           // FP32: the last finalize ran the re-sort's first phase
+    bool prep_in_finalize = true;   // FP32: run it there (else in the re-sort itself)
     const uint32_t *sorted_key = nullptr, *perm = nullptr;
     SortScratch sort;
     ResortScratch res;              // FP32: incremental re-sort
@@ -318,6 +319,9 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
     x->T = n_traps;
     x->fp32 = c->precision == DSMC_FP32_KAHAN;
     x->block = c->block_size ? c->block_size : 256;
+    // DSMC_RESORT_PREP_IN_FINALIZE=0: the re-sort's first phase runs as a
+    // kernel of the re-sort instead of in the finalize (results are the same)
+    if (const char *e = std::getenv("DSMC_RESORT_PREP_IN_FINALIZE")) x->prep_in_finalize = std::atoi(e) != 0;
     DSMC_HIP(hipGetDevice(&x->device));
     DSMC_HIP(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
 
@@ -686,8 +690,8 @@ static void fused_move(dsmc_ctx *x)
     // alive count (the removed atoms go to the end), and runs the first phase
     // of the incremental re-sort that follows
     ResortPrepArgs prep;
-    x->prepped = x->have_sort && x->cfg.sort_mode == DSMC_SORT_INCREMENTAL &&
-                 resort_supported(x->n_cap);
+    x->prepped = x->prep_in_finalize && x->have_sort &&
+                 x->cfg.sort_mode == DSMC_SORT_INCREMENTAL && resort_supported(x->n_cap);
     if (x->prepped)
         prep = resort_prep_args(x->res, n, x->block * x->apt, x->blk_moved, x->blk_dead,
                                 x->mask_moved, x->mask_dead);

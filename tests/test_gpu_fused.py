@@ -144,14 +144,18 @@ def test_incremental_sort_is_the_stable_sort():
             sim.move_atoms()
 
 
-def test_incremental_sort_equals_full_sort():
+def test_incremental_sort_equals_full_sort(monkeypatch):
     """Whole batched runs (evaporation on, graph replay, a block-size autotune
-    half way) with the incremental and the full sort are bitwise identical."""
+    half way) with the incremental and the full sort are bitwise identical,
+    with the incremental re-sort's first phase in the finalize launch or in
+    the re-sort itself."""
     run = _batch_run()
     out = {}
-    for mode in ("incremental", "full"):
+    for mode in ("incremental", "incremental-prep-in-resort", "full"):
         print(f"batched run, sort mode {mode}", flush=True)
-        with Simulation(run.with_(sort_mode=mode, block_size=64)) as sim:
+        monkeypatch.setenv("DSMC_RESORT_PREP_IN_FINALIZE",
+                           "0" if mode == "incremental-prep-in-resort" else "1")
+        with Simulation(run.with_(sort_mode=mode.split("-")[0], block_size=64)) as sim:
             sim.init_distribution()
             sim.step(20)
             sim.autotune()   # may change the move block size between steps
@@ -161,8 +165,9 @@ def test_incremental_sort_equals_full_sort():
             out[mode].append(sim.obs_log()["collisions"].copy())
             out[mode].append(sim.obs_log()["n_evap"].copy())
     assert int(out["full"][4].sum()) > 0
-    for a, b in zip(out["incremental"], out["full"]):
-        np.testing.assert_array_equal(a, b)
+    for other in ("incremental-prep-in-resort", "full"):
+        for a, b in zip(out["incremental"], out[other]):
+            np.testing.assert_array_equal(a, b)
 
 
 def test_incremental_sort_with_crowded_gap():
