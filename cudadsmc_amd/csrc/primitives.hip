@@ -264,21 +264,23 @@ __device__ __forceinline__ bool kv_less(uint32_t ka, uint32_t va, uint32_t kb, u
 __global__ __launch_bounds__(256) void k_ins_locate(
     const uint32_t *__restrict__ key_new, const uint32_t *__restrict__ key_old,
     const uint32_t *__restrict__ samp, const uint32_t *__restrict__ tsamp,
-    const uint32_t *__restrict__ mov_v, const int *__restrict__ n_dev, int *__restrict__ counts,
-    uint32_t *__restrict__ fb_k, uint32_t *__restrict__ fb_v, uint32_t *__restrict__ fb_q,
-    uint32_t *__restrict__ ov_k, uint32_t *__restrict__ ov_v, uint32_t *__restrict__ ov_q,
-    uint32_t *__restrict__ bcnt)
+    const uint32_t *__restrict__ mov_v, int n_cap, const int *__restrict__ n_dev,
+    int *__restrict__ counts, uint32_t *__restrict__ fb_k, uint32_t *__restrict__ fb_v,
+    uint32_t *__restrict__ fb_q, uint32_t *__restrict__ ov_k, uint32_t *__restrict__ ov_v,
+    uint32_t *__restrict__ ov_q, uint32_t *__restrict__ bcnt)
 {
     __shared__ uint32_t lt[kMaxBuckets];
+    // the first round's mover slot and the tile samples are loaded with the
+    // counts (entries past them are never used)
+    const int jf = blockIdx.x * 256 + threadIdx.x;
+    uint32_t vn = jf < n_cap ? mov_v[jf] : 0u;
+    const int ntc = (n_cap + kStayTile - 1) >> kStayShift;
+    for (int i = threadIdx.x; i < ntc; i += 256) lt[i] = tsamp[i];
     const int s = *n_dev, m = counts[1];
     if (blockIdx.x * 256 >= m) return;   // whole block leaves together
     const int nt = (s + kStayTile - 1) >> kStayShift;   // tiles (samples in tsamp)
     const int nw = (s + 63) >> 6;                       // 64-slot samples in samp
-    // the first round's mover is loaded while the tile samples go to LDS
-    const int jf = blockIdx.x * 256 + threadIdx.x;
-    uint32_t vn = jf < m ? mov_v[jf] : 0u;
     uint32_t kn = jf < m ? key_new[vn] : 0u;
-    for (int i = threadIdx.x; i < nt; i += 256) lt[i] = tsamp[i];
     __syncthreads();
     // the grid covers 1/16 of the slots; more movers take further rounds
     for (int j0 = 0; j0 < m; j0 += (int)gridDim.x * 256) {
@@ -665,8 +667,8 @@ void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const 
                     r.tile_cnt, r.tile_dead, r.counts, r.mov_v, r.bcnt);
     // the overflow list reuses the mover arrays that the locate does not read
     DSMC_LAUNCH(prof, "resort_locate", 0.0, k_ins_locate, div_up(div_up(r.n_cap, 16), 256), 256,
-                stream, key_new, key_old, key_samp, key_tsamp, r.mov_v, n_dev, r.counts, r.bk_k,
-                r.bk_v, r.bk_q, r.mov_k, r.mov_r, r.mov_q, r.bcnt);
+                stream, key_new, key_old, key_samp, key_tsamp, r.mov_v, r.n_cap, n_dev, r.counts,
+                r.bk_k, r.bk_v, r.bk_q, r.mov_k, r.mov_r, r.mov_q, r.bcnt);
     DSMC_LAUNCH(prof, "resort_merge", n * 12.0, k_ins_merge, div_up(r.n_cap, kStayTile) + 1, 256,
                 stream, key_new, mask_m, mask_d, n_dev, r.tile_cnt, r.tile_dead, r.bk_k, r.bk_v,
                 r.bk_q, r.mov_k, r.mov_r, r.mov_q, r.bcnt, r.counts, dead_key, r.srt_k, r.srt_v,
