@@ -675,19 +675,21 @@ struct MoveAtom {
     uint32_t kold;  // key of the last sort (before the move), when counting
 };
 
+template <bool SPIN>
 __device__ __forceinline__ void move_load(const MoveParams &mp, const FusedPtrs &A,
                                           const TrapDev &tr, uint32_t j, MoveAtom &m)
 {
     rec_load(A.r, j, m.f);
     m.id = ld_at(A.id, j);
     m.sx = m.sy = m.sz = 0.0f;
-    if (mp.spin) { m.sx = ld_at(A.s[0], j); m.sy = ld_at(A.s[1], j); m.sz = ld_at(A.s[2], j); }
+    if (SPIN) { m.sx = ld_at(A.s[0], j); m.sy = ld_at(A.s[1], j); m.sz = ld_at(A.s[2], j); }
     m.kold = 0;
 }
 
 // Spin substeps, Kahan velocity Verlet, wrap, evaporation test, next key;
 // stores the atom into slot i of B.  vals = (vx^2, vy^2, vz^2, U) of a
 // survivor; lost_e = energy of a removed atom.
+template <bool SPIN>
 __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev &tr, MoveAtom &m,
                                             long long step, int i, const FusedPtrs &B,
                                             uint32_t *__restrict__ key_out,
@@ -702,7 +704,7 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
     kahan_add(m.f.vy, m.f.rvy, a0.y * hdt);
     kahan_add(m.f.vz, m.f.rvz, a0.z * hdt);
 
-    if (mp.spin) {
+    if (SPIN) {
         const double dts = mp.dt / mp.substeps;
         const float w = (float)(mp.gs * kMuB * dts / kHbar);
         const float fdts = (float)dts;
@@ -777,7 +779,7 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
     }
     const uint32_t u = (uint32_t)i;
     rec_store_stream(B.r, u, m.f);
-    if (mp.spin) {
+    if (SPIN) {
         st_stream_at(B.s[0], u, m.sx); st_stream_at(B.s[1], u, m.sy); st_stream_at(B.s[2], u, m.sz);
     }
     st_stream_at(B.id, u, m.id);
@@ -787,8 +789,10 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
 // A block of BS threads moves APT * BS consecutive slots, APT per thread: all
 // of a thread's atoms are loaded before the first is updated, so their loads
 // are in flight together.
-template <int BS, int APT>
-__global__ __launch_bounds__(BS) void k_move_fused(
+// Without spin, two atoms per thread fit 80 registers: six waves per SIMD.
+template <int BS, int APT, bool SPIN>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(APT == 2 && !SPIN ? 6 : 1)))
+void k_move_fused(
     const TrapDev *__restrict__ traps, MoveParams mp, FusedPtrs A, FusedPtrs B,
     const uint32_t *__restrict__ perm, const int *__restrict__ trap_start,
     const int *__restrict__ n_dev, uint32_t *__restrict__ key_out, float *__restrict__ evap_pos,
@@ -830,7 +834,7 @@ __global__ __launch_bounds__(BS) void k_move_fused(
             const int i = b0 + k * BS + (int)threadIdx.x;
             at[k].t = i < n ? t_first : -1;
             if (i < n) {
-                move_load(mp, A, tr, perm ? ld_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
+                move_load<SPIN>(mp, A, tr, perm ? ld_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
                 if (mp.count_moved) at[k].kold = ld_at(mp.key_prev, (uint32_t)i);
             }
         }
@@ -838,7 +842,7 @@ __global__ __launch_bounds__(BS) void k_move_fused(
         for (int k = 0; k < APT; ++k) {
             const int i = b0 + k * BS + (int)threadIdx.x;
             if (at[k].t >= 0)
-                move_update(mp, tr, at[k], step, i, B, key_out, evap_pos, evap_vel, vals[k],
+                move_update<SPIN>(mp, tr, at[k], step, i, B, key_out, evap_pos, evap_vel, vals[k],
                             lost_e[k], lost[k], moved[k]);
         }
     } else {
@@ -850,7 +854,7 @@ __global__ __launch_bounds__(BS) void k_move_fused(
                 int t = t_first;
                 while (t < t_last && ts[t + 1] <= i) ++t;
                 at[k].t = t;
-                move_load(mp, A, traps[t], perm ? ld_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
+                move_load<SPIN>(mp, A, traps[t], perm ? ld_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
                 if (mp.count_moved) at[k].kold = ld_at(mp.key_prev, (uint32_t)i);
             }
         }
@@ -858,7 +862,7 @@ __global__ __launch_bounds__(BS) void k_move_fused(
         for (int k = 0; k < APT; ++k) {
             const int i = b0 + k * BS + (int)threadIdx.x;
             if (at[k].t >= 0)
-                move_update(mp, traps[at[k].t], at[k], step, i, B, key_out, evap_pos, evap_vel,
+                move_update<SPIN>(mp, traps[at[k].t], at[k], step, i, B, key_out, evap_pos, evap_vel,
                             vals[k], lost_e[k], lost[k], moved[k]);
         }
     }
@@ -1174,9 +1178,16 @@ static void move_fused_bs(const TrapDev *traps, const MoveParams &mp, const Fuse
                           hipStream_t st)
 {
     const size_t lds = sizeof(int) * (size_t)(mp.n_traps + 1);
-    hipLaunchKernelGGL((k_move_fused<BS, APT>), dim3(fgrid(n_cap, BS * APT)), dim3(BS), lds, st,
-                       traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos, evap_vel,
-                       rec.trap, rec.sum, rec.stride, rec.acc_over, step_dev);
+    if (mp.spin)
+        hipLaunchKernelGGL((k_move_fused<BS, APT, true>), dim3(fgrid(n_cap, BS * APT)), dim3(BS),
+                           lds, st, traps, mp, src, dst, perm, trap_start, n_dev, key_out,
+                           evap_pos, evap_vel, rec.trap, rec.sum, rec.stride, rec.acc_over,
+                           step_dev);
+    else
+        hipLaunchKernelGGL((k_move_fused<BS, APT, false>), dim3(fgrid(n_cap, BS * APT)), dim3(BS),
+                           lds, st, traps, mp, src, dst, perm, trap_start, n_dev, key_out,
+                           evap_pos, evap_vel, rec.trap, rec.sum, rec.stride, rec.acc_over,
+                           step_dev);
 }
 
 int obs_record_stride(int n_cap) { return fgrid(n_cap, 64) + 1; }
