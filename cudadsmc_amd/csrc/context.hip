@@ -140,7 +140,8 @@ struct dsmc_ctx {
     const uint32_t *sorted_key = nullptr, *perm = nullptr;
     SortScratch sort;
     ResortScratch res;              // FP32: incremental re-sort
-    double *sigvrmax = nullptr;
+    double *sigvrmax = nullptr;     // FP64 form
+    float *sigvrmax_f = nullptr;    // FP32 form: sigma v_r,max in float (half the bytes)
     dsmc_observables *obs_log = nullptr;  // (obs_cap + 1) * T records
     // device step counter, double-buffered: the kernels of a step read
     // step_buf[ssel] and its finalize writes the next value to the other one
@@ -360,7 +361,8 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
     x->val_a = x->alloc<uint32_t>(N);
     x->val_b = x->alloc<uint32_t>(N);
     sort_scratch_bind(x->sort, N, x->alloc<char>(sort_scratch_bytes(N)));
-    x->sigvrmax = x->alloc<double>(x->n_keys);
+    if (x->fp32) x->sigvrmax_f = x->alloc<float>(x->n_keys);
+    else x->sigvrmax = x->alloc<double>(x->n_keys);
     x->obs_log = x->alloc<dsmc_observables>((size_t)(x->obs_cap + 1) * n_traps);
     x->step_buf = x->alloc<long long>(2);
     x->step_save = x->alloc<long long>(1);
@@ -666,7 +668,7 @@ static void fused_collide(dsmc_ctx *x)
 {
     fused_need_table(x);
     launch_collide_fused(x->d_traps, x->d_key_base, x->T, collide_params(x), x->sorted_key,
-                         x->perm, x->n_dev + (x->cur ^ 1), x->fs[x->cur], x->sigvrmax,
+                         x->perm, x->n_dev + (x->cur ^ 1), x->fs[x->cur], x->sigvrmax_f,
                          x->coll_accum, x->step_dev(), x->n_cap, x->stream, &x->prof);
 }
 
@@ -866,7 +868,15 @@ static void reset_cells(dsmc_ctx *x)
     const long long zero = 0;
     int nn[2] = { x->n_cap, x->n_cap };
     hipStream_t st = x->stream;
-    DSMC_HIP(hipMemcpyAsync(x->sigvrmax, sv.data(), sizeof(double) * sv.size(), hipMemcpyHostToDevice, st));
+    std::vector<float> svf;
+    if (x->fp32) {
+        svf.assign(sv.begin(), sv.end());
+        DSMC_HIP(hipMemcpyAsync(x->sigvrmax_f, svf.data(), sizeof(float) * svf.size(),
+                                hipMemcpyHostToDevice, st));
+    } else {
+        DSMC_HIP(hipMemcpyAsync(x->sigvrmax, sv.data(), sizeof(double) * sv.size(),
+                                hipMemcpyHostToDevice, st));
+    }
     DSMC_HIP(hipMemcpyAsync(x->step_dev(), &zero, sizeof zero, hipMemcpyHostToDevice, st));
     DSMC_HIP(hipMemsetAsync(x->coll_accum, 0, sizeof(unsigned long long) * x->T, st));
     DSMC_HIP(hipMemcpyAsync(x->n_dev, nn, sizeof nn, hipMemcpyHostToDevice, st));
@@ -1042,7 +1052,15 @@ static void copy_field(dsmc_ctx *x, int field, void *host, size_t bytes)
         if (!x->perm) fail(DSMC_ESTATE, "no sort has run yet");
         DSMC_HIP(hipMemcpy(host, x->perm, N * 4, hipMemcpyDeviceToHost));
         return;
-    case DSMC_FIELD_SIGVRMAX: DSMC_HIP(hipMemcpy(host, x->sigvrmax, need, hipMemcpyDeviceToHost)); return;
+    case DSMC_FIELD_SIGVRMAX:
+        if (x->fp32) {
+            const std::vector<float> f = d2h(x->sigvrmax_f, (size_t)x->n_keys);
+            double *h = (double *)host;
+            for (int k = 0; k < x->n_keys; ++k) h[k] = (double)f[k];
+        } else {
+            DSMC_HIP(hipMemcpy(host, x->sigvrmax, need, hipMemcpyDeviceToHost));
+        }
+        return;
     case DSMC_FIELD_TRAP_START: {
         int *h = (int *)host;
         h[0] = 0;

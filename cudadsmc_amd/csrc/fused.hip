@@ -244,7 +244,7 @@ struct CellJob {
 // Cell of atoms [i, end) with composite key k in trap t (parameters tr).
 __device__ __forceinline__ bool cell_setup(const TrapDev &tr, int t, uint32_t k,
                                            const CollideParams &cp,
-                                           const double *__restrict__ sigvrmax, int i, int end,
+                                           const float *__restrict__ sigvrmax, int i, int end,
                                            CellJob &c)
 {
     c.i = i;
@@ -255,7 +255,7 @@ __device__ __forceinline__ bool cell_setup(const TrapDev &tr, int t, uint32_t k,
     c.t = t;
     c.cell = c.k - (uint32_t)tr.key_base;
     if ((int)c.cell >= tr.n_cells) return false;  // overflow cell: no collisions
-    c.smax = sigvrmax[c.k];
+    c.smax = (double)sigvrmax[c.k];
     const double Mc = 0.5 * (c.n - 1) * c.n;
     if (cp.pair_mode) {
         // reference pair mode: collisions.cu:318-322 including the truncation
@@ -489,14 +489,14 @@ constexpr int kWaveTrialsDefault = 16;
 // sigma v_r,max is stored when it grew.  Returns the accepted collisions.
 __device__ __forceinline__ int run_cell(const CollideParams &cp, const CellJob &c, uint32_t stream,
                                         uint32_t step, const uint32_t *__restrict__ perm,
-                                        const FusedPtrs &s, double *__restrict__ sigvrmax,
+                                        const FusedPtrs &s, float *__restrict__ sigvrmax,
                                         bool wave, const Trial0 &t0)
 {
     double smax = c.smax;
     const int cnt = wave ? cell_trials_wave(cp, c, stream, step, perm, s, smax)
                          : cell_trials_serial(cp, c, stream, step, perm, s, smax, t0);
     const bool writer = !wave || (threadIdx.x & (kWave - 1)) == 0;
-    if (writer && smax != c.smax) sigvrmax[c.k] = smax;
+    if (writer && smax != c.smax) sigvrmax[c.k] = (float)smax;
     return cnt;
 }
 
@@ -511,7 +511,7 @@ __device__ __forceinline__ int run_cell(const CollideParams &cp, const CellJob &
 __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     const TrapDev *__restrict__ traps, const int *__restrict__ key_base, int T, CollideParams cp,
     const uint32_t *__restrict__ key, const uint32_t *__restrict__ perm,
-    const int *__restrict__ n_dev, int n_cap, FusedPtrs s, double *__restrict__ sigvrmax,
+    const int *__restrict__ n_dev, int n_cap, FusedPtrs s, float *__restrict__ sigvrmax,
     unsigned long long *__restrict__ coll_accum, const long long *__restrict__ step_dev,
     int wave_trials)
 {
@@ -1146,7 +1146,7 @@ void launch_cell_range_fused(const uint32_t *key, const int *n_dev, uint32_t dea
 
 void launch_collide_fused(const TrapDev *traps, const int *key_base, int n_traps,
                           const CollideParams &cp, const uint32_t *key, const uint32_t *perm,
-                          const int *n_dev, const FusedPtrs &s, double *sigvrmax,
+                          const int *n_dev, const FusedPtrs &s, float *sigvrmax,
                           unsigned long long *coll_accum, const long long *step_dev, int n_cap,
                           hipStream_t st, Profiler *prof)
 {

@@ -124,7 +124,7 @@ void launch_cell_range_fused(const uint32_t *key, const int *n_dev, uint32_t dea
                              uint32_t *cell_start, uint32_t *cell_end, int n_cap, hipStream_t st);
 void launch_collide_fused(const TrapDev *traps, const int *key_base, int n_traps,
                           const CollideParams &cp, const uint32_t *key, const uint32_t *perm,
-                          const int *n_dev, const FusedPtrs &s, double *sigvrmax,
+                          const int *n_dev, const FusedPtrs &s, float *sigvrmax,
                           unsigned long long *coll_accum, const long long *step_dev, int n_cap,
                           hipStream_t st, Profiler *prof);
 // Per-block observable records of the fused move (see k_move_fused).
