@@ -241,10 +241,10 @@ struct CellJob {
     double   smax;    // sigma v_r,max at entry
 };
 
-// Cell of atoms [i, end) with composite key k in trap t (parameters tr).
+// Cell of atoms [i, end) with composite key k in trap t (parameters tr);
+// smax = its sigma v_r,max.
 __device__ __forceinline__ bool cell_setup(const TrapDev &tr, int t, uint32_t k,
-                                           const CollideParams &cp,
-                                           const float *__restrict__ sigvrmax, int i, int end,
+                                           const CollideParams &cp, double smax, int i, int end,
                                            CellJob &c)
 {
     c.i = i;
@@ -255,7 +255,7 @@ __device__ __forceinline__ bool cell_setup(const TrapDev &tr, int t, uint32_t k,
     c.t = t;
     c.cell = c.k - (uint32_t)tr.key_base;
     if ((int)c.cell >= tr.n_cells) return false;  // overflow cell: no collisions
-    c.smax = (double)sigvrmax[c.k];
+    c.smax = smax;
     const double Mc = 0.5 * (c.n - 1) * c.n;
     if (cp.pair_mode) {
         // reference pair mode: collisions.cu:318-322 including the truncation
@@ -573,6 +573,10 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
         nh += __popcll(bh);
         nm += __popcll(bm);
     }
+    // sigma v_r,max of this lane's first cell, loaded now (the load overlaps
+    // the end-of-chunk probe and the barrier); each round loads the next
+    // round's while it runs
+    float sm_next = lane < nm ? sigvrmax[hkey[w][multi[w][lane]]] : 0.0f;
     if (nh > 0) {
         // end of the last cell: the first later slot with another key.  The
         // wave probes the next 64 slots at once; a longer cell (e.g. the
@@ -606,6 +610,8 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     unsigned long long wave_cnt = 0;   // accepted collisions of trap t_lo (uniform chunks)
     for (int m0 = 0; m0 < nm; m0 += kWave) {
         const int mi = m0 + lane;
+        const float sm = sm_next;
+        if (mi + kWave < nm) sm_next = sigvrmax[hkey[w][multi[w][mi + kWave]]];
         CellJob c;
         Trial0 t0;
         bool ok = false;
@@ -618,7 +624,7 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
             auto setup = [&](const TrapDev &tr, int t) {
                 stream = tr.stream;
                 trial0_load(cp, stream, k - (uint32_t)tr.key_base, step, end - i, i, perm, s, t0);
-                ok = cell_setup(tr, t, k, cp, sigvrmax, i, end, c);
+                ok = cell_setup(tr, t, k, cp, (double)sm, i, end, c);
             };
             if (t_lo == t_hi) {
                 setup(traps[t_lo], t_lo);
@@ -644,7 +650,7 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
         const uint32_t k = hkey[w][h];
         const int t = t_lo == t_hi ? t_lo : last_le(kb, T + 1, (int)k);
         CellJob c;
-        (void)cell_setup(traps[t], t, k, cp, sigvrmax, i, heads[w][h + 1], c);
+        (void)cell_setup(traps[t], t, k, cp, (double)sigvrmax[k], i, heads[w][h + 1], c);
         const Trial0 none = { 0u, 0u, 0.0f, 0.0f };
         const int cnt = run_cell(cp, c, traps[t].stream, step, perm, s, sigvrmax, true, none);
         if (lane == 0) {
