@@ -1516,8 +1516,12 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
     // (threads per block, atoms per thread)
     const int cands[9][2] = { { 64, 1 }, { 128, 1 }, { 256, 1 }, { 512, 1 }, { 1024, 1 },
                               { 64, 2 }, { 128, 2 }, { 256, 2 }, { 512, 2 } };
-    double best = 1e30;
+    double best = 1e30, pref = 1e30;
     int best_bs = ctx->block, best_apt = ctx->apt;
+    // 256 threads x 2 atoms measured best in the step on MI355X; the trial
+    // moves cannot reproduce the step's cache state, so another candidate
+    // must beat it by more than 2 % to be taken
+    const int pref_bs = 256, pref_apt = 2;
     hipEvent_t b, e;
     DSMC_HIP(hipEventCreate(&b));
     DSMC_HIP(hipEventCreate(&e));
@@ -1540,7 +1544,9 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
             if (rep > 0) f = std::min(f, g);
         }
         if (f < best) { best = f; best_bs = bs; best_apt = apt; }
+        if (bs == pref_bs && apt == pref_apt) pref = f;
     }
+    if (pref <= 1.02 * best) { best_bs = pref_bs; best_apt = pref_apt; }
     ctx->prof.enabled = was;
     (void)hipEventDestroy(b);
     (void)hipEventDestroy(e);
