@@ -967,11 +967,12 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
     static_assert(kPrepWords == 256, "prep blocks are finalize blocks");
     __shared__ double red[(256 / kWave) * kAccK];
     __shared__ uint32_t ptmp[256 / kWave + 1];
+    __shared__ unsigned long long ptmp64[256 / kWave + 1];
     __shared__ bool fin;
     if ((int)blockIdx.x >= P * T) {
         resort_prep_block((int)blockIdx.x - P * T, *prep.n_dev, prep.blk_moved, prep.blk_dead,
                           prep.per_tile, prep.mask_m, prep.mask_d, prep.pre_m, prep.pre_d,
-                          prep.counts, prep.mov_v, prep.bcnt, ptmp);
+                          prep.counts, prep.mov_v, prep.bcnt, ptmp, ptmp64);
         return;
     }
     const int p = (int)blockIdx.x % P, t = (int)blockIdx.x / P;

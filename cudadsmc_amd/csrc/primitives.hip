@@ -240,8 +240,9 @@ __global__ __launch_bounds__(kPrepWords) void k_ins_prep(
     int *__restrict__ counts, uint32_t *__restrict__ mov_v, uint32_t *__restrict__ bcnt)
 {
     __shared__ uint32_t tmp[kPrepWords / kWave + 1];
+    __shared__ unsigned long long tmp64[kPrepWords / kWave + 1];
     resort_prep_block((int)blockIdx.x, *n_dev, blk_moved, blk_dead, per_tile, mask_m, mask_d, pre_m,
-                      pre_d, counts, mov_v, bcnt, tmp);
+                      pre_d, counts, mov_v, bcnt, tmp, tmp64);
 }
 
 // (key, slot) order; slots are distinct.

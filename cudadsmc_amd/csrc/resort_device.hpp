@@ -52,7 +52,8 @@ __device__ __forceinline__ void resort_prep_block(
     int k, int n, const uint32_t *__restrict__ blk_moved, const uint32_t *__restrict__ blk_dead,
     int per_tile, const uint64_t *__restrict__ mask_m, const uint64_t *__restrict__ mask_d,
     uint32_t *__restrict__ pre_m, uint32_t *__restrict__ pre_d, int *__restrict__ counts,
-    uint32_t *__restrict__ mov_v, uint32_t *__restrict__ bcnt, uint32_t *tmp)
+    uint32_t *__restrict__ mov_v, uint32_t *__restrict__ bcnt, uint32_t *tmp,
+    unsigned long long *tmp64)
 {
     const int tid = (int)threadIdx.x;
     const int words = (n + 63) >> 6;
@@ -84,14 +85,20 @@ __device__ __forceinline__ void resort_prep_block(
             }
         }
     }
+    // two block scans: the earlier blocks' totals packed as (movers, removed)
+    // in 64 bits, and the words' counts as (movers, removed) in 16-bit halves
+    // (a block holds at most 16384 slots)
+    static_assert(kPrepWords * 64 < 65536, "16-bit halves");
+    unsigned long long tot64;
+    (void)block_exclusive_scan<kPrepWords>(((unsigned long long)bm << 32) | bd, tmp64, &tot64);
+    bm = (uint32_t)(tot64 >> 32);
+    bd = (uint32_t)tot64;
     uint32_t tot;
-    (void)block_exclusive_scan<kPrepWords>(bm, tmp, &tot);
-    bm = tot;
-    (void)block_exclusive_scan<kPrepWords>(bd, tmp, &tot);
-    bd = tot;
-    uint32_t tm, td;
-    const uint32_t om = bm + block_exclusive_scan<kPrepWords>((uint32_t)__popcll(mm), tmp, &tm);
-    const uint32_t od = bd + block_exclusive_scan<kPrepWords>((uint32_t)__popcll(md), tmp, &td);
+    const uint32_t ex = block_exclusive_scan<kPrepWords>(
+        ((uint32_t)__popcll(mm) << 16) | (uint32_t)__popcll(md), tmp, &tot);
+    const uint32_t tm = tot >> 16, td = tot & 0xFFFFu;
+    const uint32_t om = bm + (ex >> 16);
+    const uint32_t od = bd + (ex & 0xFFFFu);
     if ((g & (kTileWords - 1)) == 0 && (g >> 5) < tiles) {
         pre_m[g >> 5] = om;
         pre_d[g >> 5] = od;
