@@ -339,6 +339,12 @@ __device__ __forceinline__ T ld_at(const T *base, uint32_t i)
     return *(const T *)((const char *)base + (size_t)(i * 4u));
 }
 template <typename T>
+__device__ __forceinline__ T ld_stream_at(const T *base, uint32_t i)
+{
+    static_assert(sizeof(T) == 4, "4-byte elements");
+    return __builtin_nontemporal_load((const T *)((const char *)base + (size_t)(i * 4u)));
+}
+template <typename T>
 __device__ __forceinline__ void st_stream_at(T *base, uint32_t i, T v)
 {
     static_assert(sizeof(T) == 4, "4-byte elements");

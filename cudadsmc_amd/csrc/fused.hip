@@ -685,8 +685,8 @@ template <bool SPIN>
 __device__ __forceinline__ void move_load(const MoveParams &mp, const FusedPtrs &A,
                                           const TrapDev &tr, uint32_t j, MoveAtom &m)
 {
-    rec_load(A.r, j, m.f);
-    m.id = ld_at(A.id, j);
+    rec_load_stream(A.r, j, m.f);
+    m.id = ld_stream_at(A.id, j);
     m.sx = m.sy = m.sz = 0.0f;
     if (SPIN) { m.sx = ld_at(A.s[0], j); m.sy = ld_at(A.s[1], j); m.sz = ld_at(A.s[2], j); }
     m.kold = 0;
@@ -789,7 +789,7 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
         st_stream_at(B.s[0], u, m.sx); st_stream_at(B.s[1], u, m.sy); st_stream_at(B.s[2], u, m.sz);
     }
     st_stream_at(B.id, u, m.id);
-    st_stream_at(key_out, u, kout);
+    key_out[u] = kout;   // cached: the re-sort reads the keys next
 }
 
 // A block of BS threads moves APT * BS consecutive slots, APT per thread: all
@@ -840,7 +840,7 @@ void k_move_fused(
             const int i = b0 + k * BS + (int)threadIdx.x;
             at[k].t = i < n ? t_first : -1;
             if (i < n) {
-                move_load<SPIN>(mp, A, tr, perm ? ld_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
+                move_load<SPIN>(mp, A, tr, perm ? ld_stream_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
                 if (mp.count_moved) at[k].kold = ld_at(mp.key_prev, (uint32_t)i);
             }
         }
@@ -860,7 +860,7 @@ void k_move_fused(
                 int t = t_first;
                 while (t < t_last && ts[t + 1] <= i) ++t;
                 at[k].t = t;
-                move_load<SPIN>(mp, A, traps[t], perm ? ld_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
+                move_load<SPIN>(mp, A, traps[t], perm ? ld_stream_at(perm, (uint32_t)i) : (uint32_t)i, at[k]);
                 if (mp.count_moved) at[k].kold = ld_at(mp.key_prev, (uint32_t)i);
             }
         }

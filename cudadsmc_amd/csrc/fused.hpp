@@ -53,6 +53,19 @@ __device__ __forceinline__ void rec_load(const f4 *r, uint32_t j, FState &a)
     a.rpy = p2.x; a.rpz = p2.y; a.rvy = p2.z; a.rvz = p2.w;
 }
 
+// Non-temporal: for the move's streaming read of the whole state (it is the
+// last reader of the source buffer in a step; the cache is left to the
+// re-sort's and the collide's data).
+__device__ __forceinline__ void rec_load_stream(const f4 *r, uint32_t j, FState &a)
+{
+    const f4 *q = r + rec_index(j);
+    const f4 p0 = __builtin_nontemporal_load(q), p1 = __builtin_nontemporal_load(q + kRecTile),
+             p2 = __builtin_nontemporal_load(q + 2 * kRecTile);
+    a.px = p0.x; a.py = p0.y; a.pz = p0.z; a.rpx = p0.w;
+    a.vx = p1.x; a.vy = p1.y; a.vz = p1.z; a.rvx = p1.w;
+    a.rpy = p2.x; a.rpz = p2.y; a.rvy = p2.z; a.rvz = p2.w;
+}
+
 // Non-temporal: for streaming writes of the whole state.
 __device__ __forceinline__ void rec_store_stream(f4 *r, uint32_t j, const FState &a)
 {
