@@ -66,12 +66,14 @@ __device__ __forceinline__ void rec_load_stream(const f4 *r, uint32_t j, FState 
     a.rpy = p2.x; a.rpz = p2.y; a.rvy = p2.z; a.rvz = p2.w;
 }
 
-// Non-temporal: for streaming writes of the whole state.
+// The move's write of the whole state: the position and remainder planes
+// non-temporal (only the next move reads them all), the velocity plane
+// through the cache (the next step's collide reads it for its trial pairs).
 __device__ __forceinline__ void rec_store_stream(f4 *r, uint32_t j, const FState &a)
 {
     f4 *q = r + rec_index(j);
     __builtin_nontemporal_store(f4{ a.px, a.py, a.pz, a.rpx }, q);
-    __builtin_nontemporal_store(f4{ a.vx, a.vy, a.vz, a.rvx }, q + kRecTile);
+    q[kRecTile] = f4{ a.vx, a.vy, a.vz, a.rvx };
     __builtin_nontemporal_store(f4{ a.rpy, a.rpz, a.rvy, a.rvz }, q + 2 * kRecTile);
 }
 
