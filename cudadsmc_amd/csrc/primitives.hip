@@ -215,10 +215,13 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 //      j with (key_old[j], j) >= (key, slot) (binary search of key_old, whose
 //      stays equal key_new, through samples of it that the move kernel
 //      left): the mover goes after every stay below q and before every stay
-//      from q on.  Movers are bucketed by the tile of q;
-//   4. place: movers are copied into their bucket;
-//   5. merge: one block per tile ranks its bucket's movers by counting
-//      (buckets hold a few dozen), and writes the tile's stays and movers to
+//      from q on.  Movers are bucketed by the tile of q: each takes the next
+//      place of its bucket's fixed room of kBucketCap entries (one atomic per
+//      bucket and wave), or, beyond the room, of an overflow list;
+//   4. merge: one block per tile adds up the bucket counts of the earlier
+//      tiles, ranks its bucket's movers by counting (buckets hold a few
+//      dozen; a bucket beyond its room is gathered with its overflow movers
+//      and sorted in scratch), and writes the tile's stays and movers to
 //      their final positions: with S = stays before the tile plus movers in
 //      earlier buckets, a stay goes to S + (stays of the tile before it) +
 //      (bucket movers with q <= its slot), a mover of rank r to S + (stays
@@ -229,7 +232,7 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 constexpr int kBucketLds = 1024;               // movers a bucket ranks in LDS
 constexpr int kBucketCap = kBucketLds;         // fixed room of a bucket
 constexpr int kBucketBits = 13;
-constexpr int kMaxBuckets = 1 << kBucketBits;  // LDS bucket table of the place kernel
+constexpr int kMaxBuckets = 1 << kBucketBits;  // tiles the locate's LDS sample table holds
 
 // Stand-alone first phase (when the fused finalize did not run it): see
 // resort_prep_block.
@@ -628,7 +631,7 @@ void resort_scratch_bind(ResortScratch &r, int n_cap, void *mem)
 
 bool resort_supported(int n_cap)
 {
-    // the place kernel keeps one LDS entry per tile (n / 2048 + 1 buckets)
+    // the locate keeps one LDS sample per tile (n / 2048 + 1 buckets)
     return (n_cap > 0 ? n_cap : 1) / kStayTile + 1 <= kMaxBuckets;
 }
 
