@@ -246,6 +246,10 @@ def main():
                     help="skip the rocprofv3 counter runs (roofline.traffic = null)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the step's kernels one by one instead of replaying a HIP graph")
+    ap.add_argument("--tune-cells", action="store_true",
+                    help="also time the batched step at each cells-per-trap target of the "
+                         "cells autotuner and report its choice (the measured workload keeps "
+                         "8 test particles per central cell)")
     ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
 
@@ -320,6 +324,18 @@ def main():
     if cpu:
         log(f"[bench] CPU oracle: {cpu['value']:.4e} atom-steps/s on {cpu['cores']} threads")
 
+    cells_tuning = None
+    if args.tune_cells:
+        from cudadsmc_amd import configs as _configs, tune
+        _, report = tune.autotune_cells(_configs.batched_traps())
+        best = min(report, key=lambda r: r["ms_per_step"])
+        for r in report:
+            log(f"[bench] cells target {r['target_peak']:5.2f}/central cell: {r['cells']} cells, "
+                f"block {r['block']}, {r['ms_per_step'] * 1e3:.1f} us per step")
+        cells_tuning = dict(best_target_peak=best["target_peak"],
+                            candidates=[{k: (round(v, 5) if isinstance(v, float) else v)
+                                         for k, v in r.items()} for r in report])
+
     head = result.get("batched") or result.get("fp32_single") or result.get("fp64_plain")
     steps = args.steps if "batched" in result else args.single_steps
     line = {
@@ -357,6 +373,8 @@ def main():
                                for k, v in result["fp32_single"].items()}
     if kernels:
         line["kernels"] = kernels
+    if cells_tuning:
+        line["cells_tuning"] = cells_tuning
     print(json.dumps(line), flush=True)
 
 
