@@ -106,6 +106,7 @@ PROTOTYPES = {
     "dsmc_profile_read": (I, [P, C.POINTER(KernelStat), I, C.POINTER(I)]),
     "dsmc_autotune": (I, [P, C.POINTER(I)]),
     "dsmc_sort_by_key": (I, [P, P, I, I, P]),
+    "dsmc_sort_by_key_noalloc": (I, [P, P, I, I, P]),
     "dsmc_exclusive_scan": (I, [P, P, I, P]),
     "dsmc_remove_if": (I, [P, P, I, C.POINTER(I), P]),
     "dsmc_philox": (I, [P, I, C.POINTER(C.c_uint), C.POINTER(C.c_uint), P]),

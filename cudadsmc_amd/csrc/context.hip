@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1104,6 +1105,57 @@ struct DevBuf {
 };
 
 // ---------------------------------------------------------------------------
+// Scratch of the allocation-free sort (dsmc_sort_by_key_noalloc): a
+// file-scope device array, one copy per device, resident until the process
+// ends.  Layout: the pair count (one word in a 256-byte header), the radix
+// histograms and digit totals for kNoAllocMaxPairs pairs, then the alternate
+// key and value buffers of the passes.  Host code never uses the array's
+// name as an address: the address on each device comes from
+// hipGetSymbolAddress with that device current (noalloc_scratch()).
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxPairs = 1 << 20;
+constexpr size_t kNoAllocHdr = 256;
+constexpr size_t kNoAllocSortBytes =
+    ((size_t)kRadix * (kNoAllocMaxPairs / kSortTile) * 4 + (size_t)kRadix * 4 + 256 + 255) & ~(size_t)255;
+constexpr size_t kNoAllocBytes = kNoAllocHdr + kNoAllocSortBytes + 2 * (size_t)kNoAllocMaxPairs * 4;
+static_assert(kNoAllocMaxPairs % kSortTile == 0, "whole sort tiles");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static void *g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (first call there).
+static char *noalloc_scratch()
+{
+    int dev = 0;
+    DSMC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) fail(DSMC_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        DSMC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) fail(DSMC_EHIP, "hipGetSymbolAddress returned no address for the sort scratch");
+        g_noalloc_ptr[dev] = p;
+    }
+    return (char *)g_noalloc_ptr[dev];
+}
+
+__global__ void k_store_i32(int *p, int v) { *p = v; }
+
+__global__ __launch_bounds__(256) void k_copy_pairs(uint32_t *__restrict__ dk, uint32_t *__restrict__ dv,
+                                                    const uint32_t *__restrict__ sk,
+                                                    const uint32_t *__restrict__ sv, int n)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        dk[i] = sk[i];
+        dv[i] = sv[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -1597,6 +1649,36 @@ dsmc_status dsmc_sort_by_key(unsigned int *d_keys, unsigned int *d_values, int n
         DSMC_HIP(hipMemcpyAsync(d_values, tv.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
     }
     DSMC_HIP(hipStreamSynchronize(st));
+    PRIM_END
+}
+
+dsmc_status dsmc_sort_by_key_noalloc(unsigned int *d_keys, unsigned int *d_values, int n,
+                                     int key_bits, void *stream)
+{
+    PRIM_BEGIN
+    if (n < 0 || key_bits < 1 || key_bits > 32 || (n > 0 && (!d_keys || !d_values)))
+        fail(DSMC_EINVAL, "bad sort arguments");
+    if (n > kNoAllocMaxPairs)
+        fail(DSMC_EINVAL, "dsmc_sort_by_key_noalloc sorts at most 1048576 pairs (use dsmc_sort_by_key)");
+    char *base = noalloc_scratch();
+    if (n == 0) return DSMC_OK;
+    hipStream_t st = (hipStream_t)stream;
+    int *nd = (int *)base;
+    SortScratch s;
+    sort_scratch_bind(s, n, base + kNoAllocHdr);
+    uint32_t *tk = (uint32_t *)(base + kNoAllocHdr + kNoAllocSortBytes);
+    uint32_t *tv = tk + kNoAllocMaxPairs;
+    hipLaunchKernelGGL(k_store_i32, dim3(1), dim3(1), 0, st, nd, n);
+    DSMC_HIP(hipGetLastError());
+    const int passes = key_bits <= kRadixBits ? 1 : div_up(key_bits, kRadixBits);
+    if (passes % 2 == 0) {
+        radix_sort(d_keys, d_values, tk, tv, d_keys, d_values, nd, key_bits, s, st, nullptr);
+    } else {
+        radix_sort(d_keys, d_values, d_keys, d_values, tk, tv, nd, key_bits, s, st, nullptr);
+        hipLaunchKernelGGL(k_copy_pairs, dim3(std::min(div_up(n, 256), 2048)), dim3(256), 0, st,
+                           d_keys, d_values, tk, tv, n);
+        DSMC_HIP(hipGetLastError());
+    }
     PRIM_END
 }
 

@@ -270,6 +270,21 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out);
  * LSD radix sort of n (key, value) pairs, keys in [0, 2^key_bits). */
 dsmc_status dsmc_sort_by_key(unsigned int *d_keys, unsigned int *d_values, int n,
                              int key_bits, void *stream);
+/* Allocation-free form of dsmc_sort_by_key: no counterpart in the reference.
+ * It is for callers that cannot allocate device memory, such as code under
+ * HIP graph capture, tight serving loops and harnesses that forbid
+ * hipMalloc.  Same result as dsmc_sort_by_key (stable, keys in
+ * [0, 2^key_bits)), for n <= 1048576 pairs; larger n returns DSMC_EINVAL.
+ * Its scratch is a fixed 8.5 MB device array inside the library, one copy
+ * per device, held until the process ends; no call allocates or frees device
+ * memory.  Asynchronous on `stream` (NULL: the default stream): the caller
+ * synchronises before reading the result.  Every call on one device shares
+ * that device's scratch, so calls that overlap in time on one device (other
+ * streams, other host threads) race; calls ordered on one stream do not.
+ * The first call on a device looks up the scratch's address there, so make
+ * it before any graph capture that contains the call. */
+dsmc_status dsmc_sort_by_key_noalloc(unsigned int *d_keys, unsigned int *d_values, int n,
+                                     int key_bits, void *stream);
 /* thrust::exclusive_scan<int> (main.cu:440-442). */
 dsmc_status dsmc_exclusive_scan(const int *d_in, int *d_out, int n, void *stream);
 /* thrust::remove_if(values, values + n, stencil, identity) (main.cu:493-496):
