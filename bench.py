@@ -27,7 +27,6 @@ import sqlite3
 import subprocess
 import sys
 import tempfile
-import time
 from pathlib import Path
 
 import numpy as np
@@ -37,26 +36,28 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per atom (or per pass) of each kernel of the FP32 form.
-#   move_fused: read perm (4) + 12 floats of state (48) + label (4) + the
-#               last sort's key of the slot (4); write 12 floats (48) + label
-#               (4) + next cell key (4)
-#   resort_split: read new and previous key (8), write the unchanged atoms'
-#               (key, slot) (8); resort_merge: read those (8), write the sorted
-#               (key, slot) (8) -- movers (a few percent) not counted
-#   radix_downsweep: read key + value, write key + value (first pass reads no
-#               value: identity); radix_upsweep: read key
-#   table_fused: read key; collide_fused: read key (per slot) -- per-cell work
-#               is counted by the measured traffic, not here.
-PER_ATOM_BYTES = {
-    "move_fused": 116.0,
-    "resort_split": 16.0,
+# Algorithmic bytes per atom and launch of each kernel, from the reference
+# path's own compulsory traffic (FP32 Kahan form):
+#   move: the state read and written once, x, v and their Kahan remainders,
+#         12 floats = 48 B each way (96 B); with spin the Bloch vector adds
+#         12 B each way (120 B)
+#   collide: the velocity record of each atom read and written once (24 B)
+#   sort (one logical pass over (key, slot) pairs): 16 B
+#   cell table: the sorted key read once (4 B)
+# The headline (whole-step) roofline prices one atom-step at the move's bytes.
+ALG_BYTES = {
+    "move_fused": 96.0,
+    "step_fused": 96.0,
+    "collide_fused": 24.0,
     "resort_merge": 16.0,
     "radix_downsweep": 16.0,
-    "radix_upsweep": 4.0,
     "table_fused": 4.0,
-    "collide_fused": 4.0,
 }
+SPIN_BYTES = 24.0
+# Streams the fused move (and the fused step) carry beyond that state: the
+# sort permutation (4), the atom label read and written (8), the last sort's
+# key of the slot (4) and the next cell key (4).
+FUSED_EXTRA_BYTES = 20.0
 
 
 def log(msg):
@@ -64,11 +65,16 @@ def log(msg):
 
 
 def atom_steps(sim, steps):
+    """Atoms moved in the last `steps` steps, over all traps (the survivors of
+    each step plus the atoms it removed)."""
     lg = sim.obs_log()[-steps:]
-    return int(lg["n_alive"].sum())
+    return int(lg["n_alive"].sum() + lg["n_evap"].sum())
 
 
-def run_gpu_leg(run, steps, warmup, autotune=True):
+def run_gpu_leg(run, steps, warmup, autotune=True, repeats=3):
+    """Init, autotune, warm up, then time exactly `steps` steps `repeats`
+    times (hipEvents on the context's stream, state resident in HBM); the
+    median repetition is reported."""
     from cudadsmc_amd import Simulation
 
     sim = Simulation(run)
@@ -79,9 +85,13 @@ def run_gpu_leg(run, steps, warmup, autotune=True):
         bs = sim.autotune()
     sim.step(warmup)
     sim.sync()
-    ms = sim.time_steps(steps)
-    n = atom_steps(sim, steps)
-    return sim, dict(ms=ms, atom_steps=n, value=n / (ms * 1e-3), block=bs)
+    reps = []
+    for _ in range(repeats):
+        ms = sim.time_steps(steps)
+        n = atom_steps(sim, steps)
+        reps.append(dict(ms=ms, atom_steps=n, value=n / (ms * 1e-3)))
+    med = sorted(reps, key=lambda r: r["value"])[len(reps) // 2]
+    return sim, dict(med, block=bs, repeats=[round(r["value"], 1) for r in reps])
 
 
 def energy_drift(sim, steps):
@@ -92,35 +102,39 @@ def energy_drift(sim, steps):
     return float(np.max(np.abs(E - E[0]) / np.abs(E[0])))
 
 
-def profile_leg(sim, steps):
+def profile_leg(sim, steps, spin=False):
     """Per-kernel hipEvent timing over `steps` eager steps."""
     sim.profile(True)
     sim.step(steps)
     stats = sim.profile_read()
     sim.profile(False)
     lg = sim.obs_log()[-steps:]
-    n_mean = float(lg["n_alive"].sum(axis=1).mean())
+    n_mean = float((lg["n_alive"] + lg["n_evap"]).sum(axis=1).mean())
     out = {}
     for name, st in stats.items():
         avg_ms = st["total_ms"] / max(st["launches"], 1)
         rec = dict(launches=st["launches"], total_ms=round(st["total_ms"], 4),
                    avg_us=round(avg_ms * 1e3, 3))
-        if name in PER_ATOM_BYTES:
-            b = PER_ATOM_BYTES[name] * n_mean
+        if name in ALG_BYTES:
+            per = ALG_BYTES[name] + (SPIN_BYTES if spin and name in ("move_fused", "step_fused") else 0.0)
+            b = per * n_mean
             if name == "radix_downsweep":
-                # first pass of each sort reads no values
                 passes = sim.info().radix_passes
-                b = n_mean * (16.0 * passes - 4.0) / passes
+                b = n_mean * (16.0 * passes - 4.0) / passes   # first pass reads no values
+            rec["alg_bytes_per_atom"] = per
             rec["alg_bytes"] = b
             rec["achieved_GBs"] = round(b / (avg_ms * 1e-3) / 1e9, 1)
+            if name in ("move_fused", "step_fused"):
+                rec["extra_bytes_per_atom"] = FUSED_EXTRA_BYTES
+                rec["with_extra_GBs"] = round((per + FUSED_EXTRA_BYTES) * n_mean / (avg_ms * 1e-3) / 1e9, 1)
         out[name] = rec
     return out, n_mean
 
 
 def copy_calibration(mbytes=1024, repeats=20):
-    """Achieved GB/s of a streaming copy (read + write) of `mbytes` MB: the
-    practical bandwidth ceiling of this device, and the reference kernel for
-    scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    """A streaming copy (read + write) of `mbytes` MB: a kernel of known bytes
+    for scaling the FETCH_SIZE / WRITE_SIZE counters (its rate is reported,
+    not used as a ceiling: the roofline's peak is the HBM's 8 TB/s)."""
     import ctypes as C
     from cudadsmc_amd import load_library
 
@@ -137,18 +151,56 @@ def copy_calibration(mbytes=1024, repeats=20):
     return dict(bytes=nbytes, ms=ms.value, GBs=2.0 * nbytes / (ms.value * 1e-3) / 1e9)
 
 
+GATHER_SLOTS = 1 << 24
+
+
+def gather_map(n, displaced=0.03, seed=7):
+    """Slot map of the record-gather calibration: the identity with a fraction
+    of the slots taken from up to 4096 slots away (the fused move reads ~97 %
+    of its atoms in place and the rest from near their old slots)."""
+    rng = np.random.default_rng(seed)
+    m = np.arange(n, dtype=np.int64)
+    k = rng.random(n) < displaced
+    m[k] = np.clip(m[k] + rng.integers(-4096, 4097, int(k.sum())), 0, n - 1)
+    return m.astype(np.uint32)
+
+
+def gather_calibration(n=GATHER_SLOTS, repeats=3):
+    """The record gather (dsmc_gather_copy) of n slots through gather_map(n):
+    52 B read and 48 B written per slot."""
+    import ctypes as C
+    from cudadsmc_amd import load_library
+
+    lib = load_library()
+    rec_bytes = 48 * n
+    a, b, m = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert lib.dsmc_device_alloc(C.byref(a), rec_bytes) == 0
+    assert lib.dsmc_device_alloc(C.byref(b), rec_bytes) == 0
+    assert lib.dsmc_device_alloc(C.byref(m), 4 * n) == 0
+    mp = gather_map(n)
+    assert lib.dsmc_memcpy_htod(m, mp.ctypes.data_as(C.c_void_p), 4 * n) == 0
+    ms = C.c_double()
+    st = lib.dsmc_gather_copy(b, a, m, n, repeats, C.byref(ms))
+    for p in (a, b, m):
+        lib.dsmc_device_free(p)
+    assert st == 0
+    return dict(read_bytes=52 * n, write_bytes=48 * n, ms=ms.value,
+                GBs=100.0 * n / (ms.value * 1e-3) / 1e9)
+
+
 def pmc_child(steps):
-    """Body of the counter runs: a streaming copy of known size, then `steps`
-    eager steps of the batched workload."""
+    """Body of the counter runs: a streaming copy and a record gather of
+    known sizes, then `steps` eager steps of the batched workload."""
     from cudadsmc_amd import Simulation, configs
 
     cal = copy_calibration(mbytes=1024, repeats=3)
+    gat = gather_calibration()
     run = configs.batched_traps().with_(use_graph=False, obs_capacity=steps + 64)
     with Simulation(run) as sim:
         sim.init_distribution()
         sim.step(steps)
         sim.sync()
-    print(json.dumps({"copy_bytes": cal["bytes"]}), flush=True)
+    print(json.dumps({"copy_bytes": cal["bytes"], "gather": gat}), flush=True)
 
 
 def _pmc_values(db_glob):
@@ -199,20 +251,33 @@ def pmc_traffic(steps=10, timeout_s=150):
         return v[names[0]] if names else None
 
     copy_bytes = 1024 << 20
-    f_copy, w_copy = pick(vals["FETCH_SIZE"], "stream_copy"), pick(vals["WRITE_SIZE"], "stream_copy")
-    f_move, w_move = pick(vals["FETCH_SIZE"], "move_fused"), pick(vals["WRITE_SIZE"], "move_fused")
-    if None in (f_copy, w_copy, f_move, w_move) or f_copy <= 0 or w_copy <= 0:
+    F, W = vals["FETCH_SIZE"], vals["WRITE_SIZE"]
+    f_copy, w_copy = pick(F, "stream_copy"), pick(W, "stream_copy")
+    f_gat, w_gat = pick(F, "gather_copy"), pick(W, "gather_copy")
+    if None in (f_copy, w_copy, f_gat, w_gat) or min(f_copy, w_copy, f_gat, w_gat) <= 0:
         return None
-    f_scale = copy_bytes / (f_copy * 1024.0)
-    w_scale = copy_bytes / (w_copy * 1024.0)
-    fetch = f_move * 1024.0 * f_scale
-    write = w_move * 1024.0 * w_scale
-    return dict(kernel="move_fused", fetch_bytes=fetch, write_bytes=write, bytes=fetch + write,
-                fetch_scale=round(f_scale, 4), write_scale=round(w_scale, 4),
-                fetch_kib_raw=f_move, write_kib_raw=w_move, steps=steps)
+    # counter scales: known bytes / counter bytes, for the streaming copy and
+    # for the record gather (the move's access pattern); the latter is used
+    scales = dict(copy_fetch=copy_bytes / (f_copy * 1024.0), copy_write=copy_bytes / (w_copy * 1024.0),
+                  gather_fetch=52.0 * GATHER_SLOTS / (f_gat * 1024.0),
+                  gather_write=48.0 * GATHER_SLOTS / (w_gat * 1024.0))
+    out = dict(scales={k: round(v, 4) for k, v in scales.items()}, steps=steps, kernels={})
+    for key in ("step_fused", "move_fused", "collide_fused", "resort_merge", "resort_locate",
+                "obs_finalize", "collide_edges"):
+        f, w = pick(F, key), pick(W, key)
+        if f is None or w is None:
+            continue
+        fetch = f * 1024.0 * scales["gather_fetch"]
+        write = w * 1024.0 * scales["gather_write"]
+        out["kernels"][key] = dict(fetch_bytes=round(fetch), write_bytes=round(write),
+                                   bytes=round(fetch + write), fetch_kib_raw=f, write_kib_raw=w)
+    return out
 
 
-def cpu_baseline(target_s=12.0):
+def cpu_baseline(target_s=12.0, repeats=3):
+    """The OpenMP oracle (FP64 restatement of the reference kernels) on the
+    CPU configuration, on this host's cores: `repeats` runs of about
+    target_s / repeats seconds each, the median reported."""
     from cudadsmc_amd import configs
     from oracle import oracle as orc
 
@@ -223,12 +288,77 @@ def cpu_baseline(target_s=12.0):
     sv = np.full(orc.ncells(p) + 1, orc.sigvrmax_init(p, t.T))
     probe = orc.run(p, st, 50, 0, sv)
     per_step = probe["seconds"] / 50
-    n = int(max(50, min(200000, target_s / max(per_step, 1e-6))))
-    res = orc.run(p, st, n, 50, sv)
-    return dict(value=res["atom_steps"] / res["seconds"], unit="atom-steps/s",
-                cores=res["threads"], kind="port",
+    n = int(max(50, min(200000, target_s / repeats / max(per_step, 1e-6))))
+    res = []
+    step0 = 50
+    for _ in range(repeats):
+        r = orc.run(p, st, n, step0, sv)
+        step0 += n
+        res.append(r)
+    vals = sorted(r["atom_steps"] / r["seconds"] for r in res)
+    secs = sum(r["seconds"] for r in res)
+    return dict(value=vals[len(vals) // 2], unit="atom-steps/s", cores=res[0]["threads"], kind="port",
+                runs=[round(v, 1) for v in vals],
                 sample=f"N=1e4 quadrupole, 35^3 cells, dt=1e-6 s, collisions on, FP64, "
-                       f"{n} steps ({res['seconds']:.1f} s) of the OpenMP oracle")
+                       f"{repeats} runs of {n} steps ({secs:.1f} s in all) of the OpenMP oracle; median")
+
+
+def drift_leg(run, steps=10000):
+    """max over traps of |dE/E| (alive + removed energy) over `steps` graph
+    steps from the initial state (the metric's energy condition)."""
+    from cudadsmc_amd import Simulation
+
+    with Simulation(run.with_(obs_capacity=steps + 16)) as sim:
+        sim.init_distribution()
+        sim.step(steps)
+        lg = sim.obs_log()
+    E = lg["ek"] + lg["ep"] + np.cumsum(lg["e_evap"], axis=0)
+    per_trap = np.max(np.abs(E - E[0]) / np.abs(E[0]), axis=0)
+    return dict(steps=steps, max_abs_dE_over_E=float(per_trap.max()),
+                median_trap=float(np.median(per_trap)), removed=int(lg["n_evap"].sum()))
+
+
+def precision_leg(steps=10000):
+    """The precision-form workload (N = 1e6, 100^3 cells, collisions on) run
+    for `steps` steps in the FP32 Kahan form and in the FP64 plain form from
+    one sampled state (the FP64 form's initial distribution, given to both).
+    Per result: the energy drift of each, and the FP32 form's relative
+    difference from the FP64 run in T(t) (max over steps), the cumulative
+    collision count and N.  The two runs draw the same Philox numbers, but
+    the acceptance tests round differently, so they are two samples of the
+    same process: the T and collision differences are statistical."""
+    from cudadsmc_amd import Simulation, _abi, configs
+
+    out = {}
+    logs = {}
+    with Simulation(configs.quadrupole_1e6("fp64").with_(obs_capacity=steps + 16)) as s64:
+        s64.init_distribution()
+        ids = s64.copy_out(_abi.FIELD_ATOM_ID)
+        pos = np.empty((1000000, 3))
+        vel = np.empty((1000000, 3))
+        pos[ids] = s64.copy_out(_abi.FIELD_POS)
+        vel[ids] = s64.copy_out(_abi.FIELD_VEL)
+    for prec in ("fp64", "fp32"):
+        run = configs.quadrupole_1e6(prec).with_(obs_capacity=steps + 16)
+        with Simulation(run) as sim:
+            sim.set_state(pos, vel)
+            sim.step(steps)
+            lg = sim.obs_log()[:, 0]
+        logs[prec] = lg
+        E = lg["ek"] + lg["ep"]
+        out[prec] = dict(energy_drift=float(np.max(np.abs(E - E[0]) / abs(E[0]))),
+                         collisions=int(lg["collisions"].sum()), n_final=int(lg["n_alive"][-1]))
+    T = {k: configs.MRB * v["v2"].sum(axis=1) / (3 * configs.KB * v["n_alive"]) for k, v in logs.items()}
+    c64, c32 = out["fp64"]["collisions"], out["fp32"]["collisions"]
+    out["fp32_vs_fp64"] = dict(
+        steps=steps,
+        T_rel_max=float(np.max(np.abs(T["fp32"] - T["fp64"]) / T["fp64"])),
+        T_rel_final=float(abs(T["fp32"][-1] - T["fp64"][-1]) / T["fp64"][-1]),
+        collisions_rel=float(abs(c32 - c64) / c64),
+        collisions_poisson_sigma=float(1.0 / np.sqrt(c64)),
+        N_rel=float(abs(out["fp32"]["n_final"] - out["fp64"]["n_final"]) / out["fp64"]["n_final"]),
+        energy_drift_fp32=out["fp32"]["energy_drift"], energy_drift_fp64=out["fp64"]["energy_drift"])
+    return out
 
 
 def main():
@@ -239,9 +369,13 @@ def main():
     ap.add_argument("--single-steps", type=int, default=2000)
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="batched,fp64,fp32")
+    ap.add_argument("--legs", default="batched,drift,fp64,fp32,precision,spin",
+                    help="comma list of: batched (headline), drift (1e4-step energy check of "
+                         "the batched workload), fp64 / fp32 (N = 1e6 single trap), precision "
+                         "(FP32 vs FP64 per result at N = 1e6, 1e4 steps), spin (N = 4e6 spin "
+                         "and evaporation)")
     ap.add_argument("--copy-only", action="store_true",
-                    help="only run the streaming-copy bandwidth calibration")
+                    help="only run the streaming-copy and record-gather calibration kernels")
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the rocprofv3 counter runs (roofline.traffic = null)")
     ap.add_argument("--no-graph", action="store_true",
@@ -271,22 +405,24 @@ def main():
 
     load_library()
     if args.copy_only:
-        print(json.dumps({"copy_calibration": copy_calibration()}), flush=True)
+        print(json.dumps({"stream_copy": copy_calibration(), "gather_copy": gather_calibration()}),
+              flush=True)
         return
     result = {}
 
     batched = configs.batched_traps(steps=args.steps)
-    batched.obs_capacity = args.steps + args.warmup + args.profile_steps + 64
+    batched.obs_capacity = 3 * args.steps + args.warmup + args.profile_steps + 64
     if args.no_graph:
         batched.use_graph = False
     roof = None
     kernels = None
+    step_roof = None
     if "batched" in legs:
         log(f"[bench] batched: {len(batched.traps)} traps, {batched.n_atoms} atoms")
         sim, b = run_gpu_leg(batched, args.steps, args.warmup)
         b["energy_drift"] = energy_drift(sim, args.steps)
-        log(f"[bench] batched FP32: {b['value']:.4e} atom-steps/s ({b['ms']:.1f} ms, block {b['block']}), "
-            f"max |dE/E| {b['energy_drift']:.2e}")
+        log(f"[bench] batched FP32: {b['value']:.4e} atom-steps/s (median of {b['repeats']}; "
+            f"{b['ms']:.1f} ms, block {b['block']}), max |dE/E| {b['energy_drift']:.2e}")
         result["batched"] = b
         kernels, n_mean = profile_leg(sim, args.profile_steps)
         sim.close()
@@ -294,31 +430,60 @@ def main():
                   key=lambda k: kernels[k]["total_ms"])
         d = kernels[dom]
         achieved = d["alg_bytes"] / (d["avg_us"] * 1e-6) / 1e9
-        tr = traffic if (traffic and traffic["kernel"] == dom) else None
+        tr = (traffic or {}).get("kernels", {}).get(dom)
         cal = copy_calibration()
         roof = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=round(tr["bytes"]) if tr else None,
-                    alg_bytes_per_launch=round(d["alg_bytes"]), avg_launch_us=d["avg_us"],
-                    copy_calibration_GBs=round(cal["GBs"], 1),
-                    frac_of_copy=round(achieved / cal["GBs"], 4))
-        if tr:
-            roof["traffic_detail"] = {k: (round(v, 1) if isinstance(v, float) else v)
-                                      for k, v in tr.items()}
-        log(f"[bench] dominant kernel {dom}: {achieved:.0f} GB/s")
+                    traffic=tr["bytes"] if tr else None,
+                    alg_bytes_per_atom=d["alg_bytes_per_atom"],
+                    alg_bytes_per_launch=round(d["alg_bytes"]), atoms_per_launch=round(n_mean),
+                    avg_launch_us=d["avg_us"],
+                    extra_bytes_per_atom=d.get("extra_bytes_per_atom"),
+                    with_extra_GBs=d.get("with_extra_GBs"),
+                    stream_copy_GBs=round(cal["GBs"], 1))
+        if traffic:
+            roof["traffic_detail"] = traffic
+        step_bw = b["value"] * ALG_BYTES["move_fused"] / 1e9
+        step_roof = dict(bytes_per_atom_step=ALG_BYTES["move_fused"], achieved=round(step_bw, 1),
+                         peak=HBM_PEAK_GBS, unit="GB/s", frac=round(step_bw / HBM_PEAK_GBS, 4))
+        log(f"[bench] dominant kernel {dom}: {achieved:.0f} GB/s; whole step {step_bw:.0f} GB/s")
+    if "drift" in legs:
+        result["drift"] = drift_leg(configs.batched_traps())
+        log(f"[bench] batched 1e4 steps: max |dE/E| {result['drift']['max_abs_dE_over_E']:.2e}")
     if "fp64" in legs:
-        run64 = configs.quadrupole_1e6("fp64").with_(obs_capacity=args.single_steps + args.warmup + 64)
+        run64 = configs.quadrupole_1e6("fp64").with_(obs_capacity=3 * args.single_steps + args.warmup + 64)
         sim, r = run_gpu_leg(run64, args.single_steps, args.warmup)
         sim.close()
         log(f"[bench] FP64 plain N=1e6: {r['value']:.4e} atom-steps/s")
         result["fp64_plain"] = r
     if "fp32" in legs:
-        run32 = configs.quadrupole_1e6("fp32").with_(obs_capacity=args.single_steps + args.warmup + 64)
+        run32 = configs.quadrupole_1e6("fp32").with_(obs_capacity=3 * args.single_steps + args.warmup + 64)
         sim, r = run_gpu_leg(run32, args.single_steps, args.warmup)
         r["energy_drift"] = energy_drift(sim, args.single_steps)
         sim.close()
         log(f"[bench] FP32 Kahan N=1e6: {r['value']:.4e} atom-steps/s, |dE/E| {r['energy_drift']:.2e}")
         result["fp32_single"] = r
+    if "precision" in legs:
+        result["precision"] = precision_leg()
+        log(f"[bench] FP32 vs FP64 at N=1e6: {json.dumps(result['precision']['fp32_vs_fp64'])}")
+    if "spin" in legs:
+        spin_steps = 200
+        run_s = configs.spin_evaporation("fp32").with_(
+            obs_capacity=3 * spin_steps + args.warmup + 64 + 50)
+        # the energy cut ramps over 2000 steps here instead of 2e5, so that
+        # atoms leave during the timed steps
+        run_s.traps[0].eta_steps = 2000
+        sim, r = run_gpu_leg(run_s, spin_steps, args.warmup)
+        r["energy_drift"] = energy_drift(sim, spin_steps)
+        kern, n_mean = profile_leg(sim, 50, spin=True)
+        sim.close()
+        bw = r["value"] * (ALG_BYTES["move_fused"] + SPIN_BYTES) / 1e9
+        r["step_roofline"] = dict(bytes_per_atom_step=ALG_BYTES["move_fused"] + SPIN_BYTES,
+                                  achieved=round(bw, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                                  frac=round(bw / HBM_PEAK_GBS, 4))
+        r["kernels"] = kern
+        log(f"[bench] spin + evaporation N=4e6: {r['value']:.4e} atom-steps/s, whole step {bw:.0f} GB/s")
+        result["spin"] = r
 
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:
@@ -337,6 +502,9 @@ def main():
                                          for k, v in r.items()} for r in report])
 
     head = result.get("batched") or result.get("fp32_single") or result.get("fp64_plain")
+    if head is None:
+        print(json.dumps({k: v for k, v in result.items()}), flush=True)
+        return
     steps = args.steps if "batched" in result else args.single_steps
     line = {
         "metric": "atom-steps/s (collisions on), FP32-batched vs FP64-plain at |dE/E|<=1e-4",
@@ -363,7 +531,11 @@ def main():
         "cpu_baseline": cpu,
     }
     if "batched" in result:
+        line["repeats"] = result["batched"]["repeats"]
         line["energy_drift"] = result["batched"]["energy_drift"]
+        line["step_roofline"] = step_roof
+    if "drift" in result:
+        line["energy_drift_1e4"] = result["drift"]
     if "fp64_plain" in result:
         line["fp64_plain"] = {k: round(v, 3) if isinstance(v, float) else v
                               for k, v in result["fp64_plain"].items()}
@@ -371,6 +543,11 @@ def main():
     if "fp32_single" in result:
         line["fp32_single"] = {k: (round(v, 3) if isinstance(v, float) and k != "energy_drift" else v)
                                for k, v in result["fp32_single"].items()}
+    if "precision" in result:
+        line["precision"] = result["precision"]
+    if "spin" in result:
+        line["spin"] = {k: (round(v, 3) if isinstance(v, float) and k != "energy_drift" else v)
+                        for k, v in result["spin"].items()}
     if kernels:
         line["kernels"] = kernels
     if cells_tuning:

@@ -111,6 +111,7 @@ PROTOTYPES = {
     "dsmc_remove_if": (I, [P, P, I, C.POINTER(I), P]),
     "dsmc_philox": (I, [P, I, C.POINTER(C.c_uint), C.POINTER(C.c_uint), P]),
     "dsmc_stream_copy": (I, [P, P, C.c_size_t, I, C.POINTER(C.c_double)]),
+    "dsmc_gather_copy": (I, [P, P, P, I, I, C.POINTER(C.c_double)]),
     "dsmc_device_alloc": (I, [C.POINTER(P), C.c_size_t]),
     "dsmc_device_free": (I, [P]),
     "dsmc_memcpy_htod": (I, [P, P, C.c_size_t]),

@@ -126,6 +126,8 @@ struct dsmc_ctx {
                                     // next sort's output (set by a sorted move's finalize)
     bool srt_valid = false;         // FP32: srt_k[srt_cur][i] is the key of slot i of the
                                     // current state (slots in sorted order of those keys)
+    bool dead_in_state = false;     // FP32: the current state may hold atoms a move removed
+                                    // (dead key in key_a, left in place until a sort)
     std::vector<void *> allocs;
 
     // common device buffers
@@ -138,6 +140,7 @@ struct dsmc_ctx {
     int graph_ssel = 0, graph_tsel = 0;   // step and trap-range buffers at graph capture
     bool prepped = false;           // FP32: the last finalize ran the re-sort's first phase
     bool prep_in_finalize = true;   // FP32: run it there (else in the re-sort itself)
+    bool split_step = true;         // FP32: dsmc_step runs collide and move as two kernels
     const uint32_t *sorted_key = nullptr, *perm = nullptr;
     SortScratch sort;
     ResortScratch res;              // FP32: incremental re-sort
@@ -175,6 +178,7 @@ struct dsmc_ctx {
     int *trap_start() const { return ts_buf[tsel]; }
     double *acc = nullptr, *acc2 = nullptr;
     ObsRecords rec{};
+    int *obs_first = nullptr, *obs_last = nullptr;   // on-demand observables scratch
     int max_trap_atoms = 0;
     float *evap_pos_f = nullptr, *evap_vel_f = nullptr;
     // graph of two FP32 steps (parity 0 then 1)
@@ -324,6 +328,12 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
     // DSMC_RESORT_PREP_IN_FINALIZE=0: the re-sort's first phase runs as a
     // kernel of the re-sort instead of in the finalize (results are the same)
     if (const char *e = std::getenv("DSMC_RESORT_PREP_IN_FINALIZE")) x->prep_in_finalize = std::atoi(e) != 0;
+    // DSMC_STEP_FUSED=1: dsmc_step runs the collisions inside the move kernel
+    // (the fused step) instead of the collide and move kernels one after the
+    // other; the results are bitwise the same.  The fused step is slower on
+    // MI355X (its collision code holds the kernel to four waves per SIMD,
+    // where the move alone streams at six), so it is not the default.
+    if (const char *e = std::getenv("DSMC_STEP_FUSED")) x->split_step = std::atoi(e) == 0;
     DSMC_HIP(hipGetDevice(&x->device));
     DSMC_HIP(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
 
@@ -425,6 +435,8 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
         x->rec.cnt = x->alloc<unsigned int>((size_t)n_traps);
         for (int t = 0; t < n_traps; ++t) x->max_trap_atoms = std::max(x->max_trap_atoms, traps[t].n_atoms);
         x->acc2 = x->alloc<double>((size_t)n_traps * kAccK);
+        x->obs_first = x->alloc<int>((size_t)n_traps);
+        x->obs_last = x->alloc<int>((size_t)n_traps);
         x->evap_pos_f = x->alloc<float>(3 * (size_t)N);
         x->evap_vel_f = x->alloc<float>(3 * (size_t)N);
     }
@@ -620,7 +632,7 @@ static MoveParams move_params(const dsmc_ctx *x)
 static void fused_index(dsmc_ctx *x)
 {
     launch_index_fused(x->d_traps, x->trap_start(), x->T, x->fs[x->cur], x->key_a, x->n_dev + x->cur,
-                       x->n_cap, x->stream, &x->prof);
+                       x->n_cap, x->dead_key, x->dead_in_state, x->stream, &x->prof);
     x->tbl_valid = false;   // the new keys decide the trap ranges of the next sort
 }
 
@@ -673,8 +685,15 @@ static void fused_collide(dsmc_ctx *x)
                          x->coll_accum, x->step_dev(), x->n_cap, x->stream, &x->prof);
 }
 
-static void fused_move(dsmc_ctx *x)
+// coll: the fused step (the collisions of this step run inside the move
+// kernel, those of cells across its block boundaries in an edge pass before
+// it); needs a pending sort.
+static void fused_move(dsmc_ctx *x, bool coll = false)
 {
+    // atoms removed by the previous move are dropped by a sort: a second move
+    // without one would move (and test) them again
+    if (!x->have_sort && x->dead_in_state)
+        fail(DSMC_ESTATE, "the last move removed atoms: sort before moving again");
     fused_need_table(x);
     // after a sort the alive count of the sorted order is n_dev[cur ^ 1]
     const int *n = x->have_sort ? x->n_dev + (x->cur ^ 1) : x->n_dev + x->cur;
@@ -685,10 +704,21 @@ static void fused_move(dsmc_ctx *x)
         mp.count_moved = 1;
         mp.key_prev = x->sorted_key;
     }
+    StepCollide sc;
+    if (coll) {
+        if (!x->have_sort) fail(DSMC_ESTATE, "the fused step needs a sorted state");
+        sc.cp = collide_params(x);
+        sc.sigvrmax = x->sigvrmax_f;
+        sc.coll_accum = x->coll_accum;
+        sc.dry = 0;
+        launch_collide_edges(x->d_traps, x->d_key_base, x->T, sc.cp, x->sorted_key, x->perm, n,
+                             x->fs[x->cur], x->sigvrmax_f, x->coll_accum, x->step_dev(),
+                             64 * x->apt, x->n_cap, x->stream, &x->prof);   // one wave's span
+    }
     launch_move_fused(x->block, x->apt, x->d_traps, mp, x->fs[x->cur], x->fs[x->cur ^ 1],
                       x->have_sort ? x->perm : nullptr, x->trap_start(), n, x->key_a,
                       x->evap_pos_f, x->evap_vel_f, x->rec, x->step_dev(), x->n_cap, x->stream,
-                      &x->prof);
+                      &x->prof, coll ? &sc : nullptr);
     // a sorted move's finalize also leaves the next sort's trap ranges and
     // alive count (the removed atoms go to the end), and runs the first phase
     // of the incremental re-sort that follows
@@ -714,15 +744,21 @@ static void fused_move(dsmc_ctx *x)
     x->srt_valid = x->have_sort;
     x->tbl_valid = x->have_sort;
     x->have_sort = false;
+    x->dead_in_state = x->cfg.evaporation != 0;
     x->cur ^= 1;
 }
 
 static void fused_step(dsmc_ctx *x)
 {
     fused_sort(x);
-    fused_collide(x);
+    if (x->split_step) {   // the collide and the move as two kernels
+        fused_collide(x);
+        fused_move(x);
+    } else {
+        fused_need_table(x);
+        fused_move(x, true);
+    }
     x->coll_steps++;
-    fused_move(x);
     x->step++;
 }
 
@@ -787,48 +823,50 @@ static void do_steps(dsmc_ctx *x, int n)
     for (; k < n; ++k) fused_step(x);
 }
 
-// Sync and read the current alive count.
-static int host_alive(dsmc_ctx *x)
+// Sync and read the number of slots of the current state buffer (FP32: alive
+// atoms plus the atoms the last move removed, which keep their slots until
+// the next sort).
+static int host_slots(dsmc_ctx *x)
 {
     int n = 0;
     DSMC_HIP(hipStreamSynchronize(x->stream));
-    const int *src = x->fp32 ? x->n_dev + (x->have_sort ? (x->cur ^ 1) : x->cur) : x->n_dev;
-    DSMC_HIP(hipMemcpy(&n, src, sizeof(int), hipMemcpyDeviceToHost));
+    DSMC_HIP(hipMemcpy(&n, x->fp32 ? x->n_dev + x->cur : x->n_dev, sizeof(int), hipMemcpyDeviceToHost));
     return n;
+}
+
+// FP32 form: the alive slots of the current state in slot order (all slots
+// when no move has removed atoms since the last sort); returns the slot count.
+static int fused_live(dsmc_ctx *x, std::vector<uint32_t> &live, std::vector<uint32_t> *keys = nullptr)
+{
+    const int n = host_slots(x);
+    live.clear();
+    std::vector<uint32_t> k;
+    if (x->dead_in_state || keys) {
+        k.resize((size_t)n);
+        if (n) DSMC_HIP(hipMemcpy(k.data(), x->key_a, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost));
+    }
+    live.reserve((size_t)n);
+    for (int i = 0; i < n; ++i)
+        if (!x->dead_in_state || k[(size_t)i] != x->dead_key) live.push_back((uint32_t)i);
+    if (keys) {
+        keys->resize(live.size());
+        for (size_t j = 0; j < live.size(); ++j) (*keys)[j] = k[live[j]];
+    }
+    return n;
+}
+
+// Sync and read the current alive count.
+static int host_alive(dsmc_ctx *x)
+{
+    if (!x->fp32) return host_slots(x);
+    if (!x->dead_in_state) return host_slots(x);
+    std::vector<uint32_t> live;
+    (void)fused_live(x, live);
+    return (int)live.size();
 }
 
 // Clears one 64-bit counter (a kernel, not a memset: callers may capture it).
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
-
-// On-demand FP32 observables of the current state into log row obs_cap.
-__global__ void k_observe_fused(const TrapDev *__restrict__ traps, const int *__restrict__ trap_start,
-                                int T, double mass, FusedPtrs s, const int *__restrict__ n_dev,
-                                double *__restrict__ acc)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *n_dev) return;
-    int lo = 0, hi = T - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (trap_start[mid] <= i) lo = mid; else hi = mid - 1;
-    }
-    const TrapDev &tr = traps[lo];
-    FState f;
-    rec_load(s.r, (uint32_t)i, f);
-    const double X = (double)f.px + (double)f.rpx;
-    const double Y = (double)f.py + (double)f.rpy;
-    const double Z = (double)f.pz + (double)f.rpz;
-    const double VX = (double)f.vx + (double)f.rvx;
-    const double VY = (double)f.vy + (double)f.rvy;
-    const double VZ = (double)f.vz + (double)f.rvz;
-    double *a = acc + (size_t)lo * kAccK;
-    atomicAdd(&a[kAccEk], 0.5 * mass * (VX * VX + VY * VY + VZ * VZ));
-    atomicAdd(&a[kAccEp], potential_energy(tr, X, Y, Z));
-    atomicAdd(&a[kAccV2x], VX * VX);
-    atomicAdd(&a[kAccV2y], VY * VY);
-    atomicAdd(&a[kAccV2z], VZ * VZ);
-    atomicAdd(&a[kAccN], 1.0);
-}
 
 static void observe_now(dsmc_ctx *x, std::vector<dsmc_observables> &out)
 {
@@ -840,12 +878,9 @@ static void observe_now(dsmc_ctx *x, std::vector<dsmc_observables> &out)
                            hipMemcpyDeviceToHost));
         return;
     }
-    if (x->have_sort) fail(DSMC_ESTATE, "a sort is pending: call dsmc_move_atoms first");
-    DSMC_HIP(hipMemsetAsync(x->acc2, 0, sizeof(double) * kAccK * x->T, x->stream));
-    hipLaunchKernelGGL(k_observe_fused, dim3(div_up(x->n_cap, 256)), dim3(256), 0, x->stream,
-                       x->d_traps, x->trap_start(), x->T, x->cfg.mass, x->fs[x->cur],
-                       x->n_dev + x->cur, x->acc2);
-    DSMC_HIP(hipGetLastError());
+    launch_observe_fused(x->d_traps, x->d_key_base, x->T, x->cfg.mass, x->fs[x->cur], x->key_a,
+                         x->n_dev + x->cur, x->n_cap, x->dead_key, x->obs_first, x->obs_last,
+                         x->acc2, x->stream);
     std::vector<double> a((size_t)x->T * kAccK);
     DSMC_HIP(hipStreamSynchronize(x->stream));
     DSMC_HIP(hipMemcpy(a.data(), x->acc2, sizeof(double) * a.size(), hipMemcpyDeviceToHost));
@@ -890,6 +925,7 @@ static void reset_cells(dsmc_ctx *x)
     x->cur = 0;
     x->have_sort = false;
     x->srt_valid = false;
+    x->dead_in_state = false;
     x->tbl_valid = true;    // all atoms alive, traps at their initial ranges
 }
 
@@ -955,9 +991,25 @@ static size_t field_size(dsmc_ctx *x, int field, int n)
     }
 }
 
+// Trap ranges of the alive atoms in slot order (traps are contiguous there),
+// from their composite keys.
+static void trap_ranges_of(const dsmc_ctx *x, const std::vector<uint32_t> &keys, int *out)
+{
+    std::vector<int> cnt(x->T, 0);
+    for (uint32_t k : keys) {
+        const int t = (int)(std::upper_bound(x->key_base.begin(), x->key_base.begin() + x->T, (int)k) -
+                            x->key_base.begin()) - 1;
+        cnt[t]++;
+    }
+    out[0] = 0;
+    for (int t = 0; t < x->T; ++t) out[t + 1] = out[t] + cnt[t];
+}
+
 static void copy_field(dsmc_ctx *x, int field, void *host, size_t bytes)
 {
-    const int n = host_alive(x);
+    std::vector<uint32_t> live;
+    const int n_slots = x->fp32 ? fused_live(x, live) : host_slots(x);
+    const int n = x->fp32 ? (int)live.size() : n_slots;
     const size_t need = field_size(x, field, n);
     if (need == 0) fail(DSMC_ENOTSUP, "field not available in this form");
     if (bytes < need) fail(DSMC_EINVAL, "host buffer too small");
@@ -997,12 +1049,12 @@ static void copy_field(dsmc_ctx *x, int field, void *host, size_t bytes)
         case DSMC_FIELD_POS: case DSMC_FIELD_VEL: case DSMC_FIELD_POS_COMP: case DSMC_FIELD_VEL_COMP: {
             const bool pos = field == DSMC_FIELD_POS || field == DSMC_FIELD_POS_COMP;
             const bool comp = field == DSMC_FIELD_POS_COMP || field == DSMC_FIELD_VEL_COMP;
-            const std::vector<f4> rec = d2h(s.r, rec_elems(N));
+            const std::vector<f4> rec = d2h(s.r, rec_elems((size_t)n_slots));
             for (int c = 0; c < 3; ++c) {
                 const int *h = pos ? kRecPosHi[c] : kRecVelHi[c], *l = pos ? kRecPosLo[c] : kRecVelLo[c];
                 cols[c].resize(N);
                 for (size_t i = 0; i < N; ++i) {
-                    const f4 *q = rec.data() + rec_index((uint32_t)i);
+                    const f4 *q = rec.data() + rec_index(live[i]);
                     const float a = q[h[0] * kRecTile][h[1]], b = q[l[0] * kRecTile][l[1]];
                     cols[c][i] = comp ? (double)b : (double)a + (double)b;
                 }
@@ -1010,11 +1062,17 @@ static void copy_field(dsmc_ctx *x, int field, void *host, size_t bytes)
             interleave(hd, cols, 3, N);
             return;
         }
-        case DSMC_FIELD_ATOM_ID: DSMC_HIP(hipMemcpy(host, s.id, N * 4, hipMemcpyDeviceToHost)); return;
+        case DSMC_FIELD_ATOM_ID: {
+            const std::vector<int> a = d2h(s.id, (size_t)n_slots);
+            int *hi_ = (int *)host;
+            for (size_t i = 0; i < N; ++i) hi_[i] = a[live[i]];
+            return;
+        }
         case DSMC_FIELD_SPIN:
             for (int c = 0; c < 3; ++c) {
-                std::vector<float> a = d2h(s.s[c], N);
-                cols[c].assign(a.begin(), a.end());
+                const std::vector<float> a = d2h(s.s[c], (size_t)n_slots);
+                cols[c].resize(N);
+                for (size_t i = 0; i < N; ++i) cols[c][i] = a[live[i]];
             }
             interleave(hd, cols, 3, N);
             return;
@@ -1037,13 +1095,27 @@ static void copy_field(dsmc_ctx *x, int field, void *host, size_t bytes)
             for (int k = 0; k < x->n_keys; ++k) { hi_[2 * k] = (int)a[k]; hi_[2 * k + 1] = (int)b[k] - 1; }
             return;
         }
-        case DSMC_FIELD_TRAP_START:
-            DSMC_HIP(hipMemcpy(host, x->trap_start(), need, hipMemcpyDeviceToHost)); return;
+        case DSMC_FIELD_TRAP_START: {
+            std::vector<uint32_t> lk;
+            (void)fused_live(x, live, &lk);
+            trap_ranges_of(x, lk, (int *)host);
+            return;
+        }
+        case DSMC_FIELD_CELL_ID:
+            if (!x->have_sort) {   // keys of the alive slots (unsorted order)
+                const std::vector<uint32_t> a = d2h(x->key_a, (size_t)n_slots);
+                uint32_t *hk = (uint32_t *)host;
+                for (size_t i = 0; i < N; ++i) hk[i] = a[live[i]];
+                return;
+            }
+            break;
         default: break;
         }
     }
     switch (field) {
     case DSMC_FIELD_CELL_ID: {
+        // FP64 form: the keys of the last index or sort; FP32 form after a
+        // sort: the sorted keys (the alive atoms; removed atoms sort last)
         const uint32_t *src = (x->fp32 && !x->have_sort) ? x->key_a : x->sorted_key;
         if (!src) src = x->key_a;
         DSMC_HIP(hipMemcpy(host, src, N * 4, hipMemcpyDeviceToHost));
@@ -1281,13 +1353,14 @@ dsmc_status dsmc_set_spin(dsmc_ctx *ctx, const double *spin, int n, int componen
         DSMC_HIP(hipStreamSynchronize(ctx->stream));
     } else {
         if (components != 3) fail(DSMC_EINVAL, "the FP32 form stores Bloch vectors (3 components)");
-        std::vector<float> col(3 * N);
+        // rows go to the alive slots; slots of removed atoms keep their values
+        std::vector<uint32_t> live;
+        const size_t S = (size_t)fused_live(ctx, live);
         for (int c = 0; c < 3; ++c) {
-            for (size_t i = 0; i < N; ++i) col[c * N + i] = (float)spin[3 * i + c];
-            DSMC_HIP(hipMemcpyAsync(ctx->fs[ctx->cur].s[c], col.data() + c * N, N * 4,
-                                    hipMemcpyHostToDevice, ctx->stream));
+            std::vector<float> col = d2h(ctx->fs[ctx->cur].s[c], S);
+            for (size_t i = 0; i < N; ++i) col[live[i]] = (float)spin[3 * i + c];
+            DSMC_HIP(hipMemcpy(ctx->fs[ctx->cur].s[c], col.data(), S * 4, hipMemcpyHostToDevice));
         }
-        DSMC_HIP(hipStreamSynchronize(ctx->stream));
     }
     DSMC_API_END(ctx)
 }
@@ -1579,6 +1652,14 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
     DSMC_HIP(hipEventCreate(&e));
     const bool was = ctx->prof.enabled;
     ctx->prof.enabled = false;
+    // on sorted state the trials time the fused step kernel (its collisions
+    // computed but not stored), as the step loop runs it
+    StepCollide sc;
+    sc.cp = collide_params(ctx);
+    sc.sigvrmax = ctx->sigvrmax_f;
+    sc.coll_accum = ctx->coll_accum;
+    sc.dry = 1;
+    const StepCollide *coll = (ctx->srt_valid && !ctx->split_step) ? &sc : nullptr;
     for (const auto &cand : cands) {
         const int bs = cand[0], apt = cand[1];
         float f = 1e30f;
@@ -1589,7 +1670,7 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
                 launch_move_fused(bs, apt, ctx->d_traps, mp, ctx->fs[ctx->cur], ctx->fs[ctx->cur ^ 1],
                                   nullptr, ctx->trap_start(), ctx->n_dev + ctx->cur, ctx->res.alt_k,
                                   ctx->evap_pos_f, ctx->evap_vel_f, ctx->rec, ctx->step_dev(),
-                                  ctx->n_cap, ctx->stream, nullptr);
+                                  ctx->n_cap, ctx->stream, nullptr, coll);
             DSMC_HIP(hipEventRecord(e, ctx->stream));
             DSMC_HIP(hipEventSynchronize(e));
             DSMC_HIP(hipEventElapsedTime(&g, b, e));
@@ -1736,6 +1817,28 @@ dsmc_status dsmc_stream_copy(void *d_dst, const void *d_src, size_t bytes, int r
     stream_copy(d_dst, d_src, bytes, nullptr);  // warm-up
     DSMC_HIP(hipEventRecord(b, nullptr));
     for (int r = 0; r < repeats; ++r) stream_copy(d_dst, d_src, bytes, nullptr);
+    DSMC_HIP(hipEventRecord(e, nullptr));
+    DSMC_HIP(hipEventSynchronize(e));
+    float f = 0.f;
+    DSMC_HIP(hipEventElapsedTime(&f, b, e));
+    (void)hipEventDestroy(b);
+    (void)hipEventDestroy(e);
+    if (ms) *ms = f / repeats;
+    PRIM_END
+}
+
+dsmc_status dsmc_gather_copy(void *d_dst, const void *d_src, const unsigned int *d_map, int n_slots,
+                             int repeats, double *ms)
+{
+    PRIM_BEGIN
+    if (!d_dst || !d_src || !d_map || n_slots < 1 || repeats < 1)
+        fail(DSMC_EINVAL, "bad gather_copy arguments");
+    hipEvent_t b, e;
+    DSMC_HIP(hipEventCreate(&b));
+    DSMC_HIP(hipEventCreate(&e));
+    gather_copy(d_dst, d_src, d_map, n_slots, nullptr);  // warm-up
+    DSMC_HIP(hipEventRecord(b, nullptr));
+    for (int r = 0; r < repeats; ++r) gather_copy(d_dst, d_src, d_map, n_slots, nullptr);
     DSMC_HIP(hipEventRecord(e, nullptr));
     DSMC_HIP(hipEventSynchronize(e));
     float f = 0.f;

@@ -156,9 +156,12 @@ void gather_u32(uint32_t *dst, const uint32_t *src, const uint32_t *map, const i
 void fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t stream, Profiler *prof);
 void fill_f64(double *p, double v, long long n, hipStream_t stream, Profiler *prof);
 
-// Streaming device-to-device copy of bytes (multiple of 16) for bandwidth
+// Streaming device-to-device copy of bytes (multiple of 16) for counter
 // calibration.
 void stream_copy(void *dst, const void *src, size_t bytes, hipStream_t stream);
+// Gather of n slots of the FP32 state's tiled records (48 bytes a slot, in
+// whole 64-slot tiles) through map, for counter calibration.
+void gather_copy(void *dst, const void *src, const uint32_t *map, int n, hipStream_t stream);
 
 void philox_fill(uint32_t *out, int n, const uint32_t ctr[4], const uint32_t key[2],
                  hipStream_t stream);

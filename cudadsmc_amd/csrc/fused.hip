@@ -22,7 +22,9 @@
 #include "fused.hpp"
 #include "resort_device.hpp"
 
+#include <algorithm>
 #include <cstdlib>
+#include <cstring>
 
 namespace dsmc {
 
@@ -30,6 +32,8 @@ constexpr int kMaxTrapsLds = 8192;
 int fused_max_traps() { return kMaxTrapsLds; }
 
 constexpr int kFBlock = 256;
+
+static int fgrid(long long n, int bs) { return div_up(n > 0 ? n : 1, bs); }
 
 // Largest t in [0, n) with arr[t] <= v (arr ascending, arr[0] <= v).
 __device__ __forceinline__ int last_le(const int *arr, int n, int v)
@@ -108,17 +112,21 @@ __global__ __launch_bounds__(kFBlock) void k_init_fused(const TrapDev *__restric
     s.id[a] = a;
 }
 
-// indexAtoms for the FP32 state: the FP64 index of p + cp.
+// indexAtoms for the FP32 state: the FP64 index of p + cp.  With keep_dead, a
+// slot that already holds the dead key (an atom removed by the last move,
+// which stays in place until the next sort drops it) keeps it.
 __global__ __launch_bounds__(kFBlock) void k_index_fused(const TrapDev *__restrict__ traps,
                                                          const int *__restrict__ trap_start,
                                                          int T, FusedPtrs s,
                                                          uint32_t *__restrict__ key,
-                                                         const int *__restrict__ n_dev)
+                                                         const int *__restrict__ n_dev,
+                                                         uint32_t dead_key, int keep_dead)
 {
     extern __shared__ int ts[];
     load_table(ts, trap_start, T + 1);
     const int i = blockIdx.x * kFBlock + threadIdx.x;
     if (i >= *n_dev) return;
+    if (keep_dead && key[i] == dead_key) return;
     const int t = last_le(ts, T, i);
     const TrapDev &tr = traps[t];
     const f4 *q = s.r + rec_index((uint32_t)i);
@@ -312,9 +320,8 @@ __device__ __forceinline__ f2 *vel_rem_yz(f4 *r, uint32_t j)
     return (f2 *)(r + rec_index(j) + 2 * kRecTile) + 1;
 }
 
-__device__ __forceinline__ float rel_speed(const FusedPtrs &s, uint32_t a, uint32_t b)
+__device__ __forceinline__ float rel_speed_v(const f4 va, const f4 vb)
 {
-    const f4 va = *vel_rec(s.r, a), vb = *vel_rec(s.r, b);
     const float dx = va.x - vb.x;
     const float dy = va.y - vb.y;
     const float dz = va.z - vb.z;
@@ -353,15 +360,11 @@ __device__ __forceinline__ void sphere_point_f(unsigned long long seed, uint32_t
 }
 
 // Accepted collision of atoms a and b of trial tr, in FP64 from value +
-// remainder, stored back as float + remainder (collisions.cu:426-436).
-__device__ __forceinline__ void scatter_pair(const CollideParams &cp, uint32_t stream, uint32_t cell,
-                                             uint32_t step, int tr, const FusedPtrs &s, uint32_t a,
-                                             uint32_t b)
+// remainder, returned as float + remainder (collisions.cu:426-436): va, vb =
+// (vx, vy, vz, rvx), wa, wb = (rvy, rvz).
+__device__ __forceinline__ void scatter_vals(const CollideParams &cp, uint32_t stream, uint32_t cell,
+                                             uint32_t step, int tr, f4 &va, f2 &wa, f4 &vb, f2 &wb)
 {
-    f4 *const ha = vel_rec(s.r, a), *const hb = vel_rec(s.r, b);
-    f2 *const za = vel_rem_yz(s.r, a), *const zb = vel_rem_yz(s.r, b);
-    const f4 va = *ha, vb = *hb;   // (vx, vy, vz, rvx)
-    const f2 wa = *za, wb = *zb;   // (rvy, rvz)
     const double vax = (double)va.x + (double)va.w;
     const double vay = (double)va.y + (double)wa.x;
     const double vaz = (double)va.z + (double)wa.y;
@@ -381,11 +384,59 @@ __device__ __forceinline__ void scatter_pair(const CollideParams &cp, uint32_t s
     split_df(cx + 0.5 * wx, nb.vx, nb.rvx);
     split_df(cy + 0.5 * wy, nb.vy, nb.rvy);
     split_df(cz + 0.5 * wz, nb.vz, nb.rvz);
-    *ha = f4{ na.vx, na.vy, na.vz, na.rvx };
-    *za = f2{ na.rvy, na.rvz };
-    *hb = f4{ nb.vx, nb.vy, nb.vz, nb.rvx };
-    *zb = f2{ nb.rvy, nb.rvz };
+    va = f4{ na.vx, na.vy, na.vz, na.rvx };
+    wa = f2{ na.rvy, na.rvz };
+    vb = f4{ nb.vx, nb.vy, nb.vz, nb.rvx };
+    wb = f2{ nb.rvy, nb.rvz };
 }
+
+// Where the trials of a cell find the atoms' velocities.  A cell's atoms are
+// its slots c.i .. c.i + c.n - 1 in sorted order; slot() maps a sorted slot
+// to the index the velocity accessors take.
+//   VelGlobal: the state records in HBM, atom of sorted slot j = perm[j]
+//              (the state is in the previous order until the move gathers it);
+//   VelLds:    the fused step's block copy in LDS, index = slot - block start.
+struct VelGlobal {
+    f4 *r;
+    const uint32_t *perm;
+    __device__ __forceinline__ uint32_t slot(int j) const { return perm[j]; }
+    __device__ __forceinline__ f4 vel(uint32_t a) const { return *vel_rec(r, a); }
+    __device__ __forceinline__ void scatter(const CollideParams &cp, uint32_t stream, uint32_t cell,
+                                            uint32_t step, int tr, uint32_t a, uint32_t b) const
+    {
+        f4 *const ha = vel_rec(r, a), *const hb = vel_rec(r, b);
+        f2 *const za = vel_rem_yz(r, a), *const zb = vel_rem_yz(r, b);
+        f4 va = *ha, vb = *hb;
+        f2 wa = *za, wb = *zb;
+        scatter_vals(cp, stream, cell, step, tr, va, wa, vb, wb);
+        *ha = va; *za = wa;
+        *hb = vb; *zb = wb;
+    }
+    // the wave's next round reads what one lane wrote
+    __device__ __forceinline__ void publish() const { __threadfence(); }
+};
+
+struct VelLds {
+    f4 *v;
+    f2 *w;
+    int b0;
+    __device__ __forceinline__ uint32_t slot(int j) const { return (uint32_t)(j - b0); }
+    __device__ __forceinline__ f4 vel(uint32_t a) const { return v[a]; }
+    __device__ __forceinline__ void scatter(const CollideParams &cp, uint32_t stream, uint32_t cell,
+                                            uint32_t step, int tr, uint32_t a, uint32_t b) const
+    {
+        f4 va = v[a], vb = v[b];
+        f2 wa = w[a], wb = w[b];
+        scatter_vals(cp, stream, cell, step, tr, va, wa, vb, wb);
+        v[a] = va; w[a] = wa;
+        v[b] = vb; w[b] = wb;
+    }
+    __device__ __forceinline__ void publish() const
+    {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+};
 
 // The first trial of a cell (its pair, acceptance uniform and relative
 // speed), loaded before the cell's trial count is known: the count needs
@@ -395,24 +446,22 @@ struct Trial0 {
     float u, vr;
 };
 
+template <class V>
 __device__ __forceinline__ void trial0_load(const CollideParams &cp, uint32_t stream, uint32_t cell,
-                                            uint32_t step, int n, int i,
-                                            const uint32_t *__restrict__ perm, const FusedPtrs &s,
-                                            Trial0 &t)
+                                            uint32_t step, int n, int i, const V &vs, Trial0 &t)
 {
     int ia, ib;
     trial_draw(cp, stream, cell, step, 0, n, ia, ib, t.u);
-    t.a = perm[i + ia];
-    t.b = perm[i + ib];
-    t.vr = rel_speed(s, t.a, t.b);
+    t.a = vs.slot(i + ia);
+    t.b = vs.slot(i + ib);
+    t.vr = rel_speed_v(vs.vel(t.a), vs.vel(t.b));
 }
 
 // The trials of one cell in sequence, by one lane (collisions.cu:397-441).
+template <class V>
 __device__ __forceinline__ int cell_trials_serial(const CollideParams &cp, const CellJob &c,
-                                                  uint32_t stream, uint32_t step,
-                                                  const uint32_t *__restrict__ perm,
-                                                  const FusedPtrs &s, double &smax,
-                                                  const Trial0 &t0)
+                                                  uint32_t stream, uint32_t step, const V &vs,
+                                                  double &smax, const Trial0 &t0)
 {
     int count = 0;
     for (int tr = 0; tr < c.ncol; ++tr) {
@@ -421,14 +470,14 @@ __device__ __forceinline__ int cell_trials_serial(const CollideParams &cp, const
         if (tr > 0) {
             int ia, ib;
             trial_draw(cp, stream, c.cell, step, tr, c.n, ia, ib, u);
-            a = perm[c.i + ia];
-            b = perm[c.i + ib];
-            vr = rel_speed(s, a, b);
+            a = vs.slot(c.i + ia);
+            b = vs.slot(c.i + ib);
+            vr = rel_speed_v(vs.vel(a), vs.vel(b));
         }
         const double sv = (double)vr * cp.sigma;
         if (sv > smax) smax = sv;
         if (c.pc * vr > u) {
-            scatter_pair(cp, stream, c.cell, step, tr, s, a, b);
+            vs.scatter(cp, stream, c.cell, step, tr, a, b);
             ++count;
         }
     }
@@ -440,10 +489,10 @@ __device__ __forceinline__ int cell_trials_serial(const CollideParams &cp, const
 // one they are exactly the sequential trials (no velocity changed before
 // them); that collision is applied and evaluation resumes after it.  All 64
 // lanes call this with the same job.
+template <class V>
 __device__ __forceinline__ int cell_trials_wave(const CollideParams &cp, const CellJob &c,
-                                                uint32_t stream, uint32_t step,
-                                                const uint32_t *__restrict__ perm,
-                                                const FusedPtrs &s, double &smax)
+                                                uint32_t stream, uint32_t step, const V &vs,
+                                                double &smax)
 {
     const int lane = threadIdx.x & (kWave - 1);
     int count = 0;
@@ -458,9 +507,9 @@ __device__ __forceinline__ int cell_trials_wave(const CollideParams &cp, const C
             int ia, ib;
             float u;
             trial_draw(cp, stream, c.cell, step, tr, c.n, ia, ib, u);
-            a = perm[c.i + ia];
-            b = perm[c.i + ib];
-            const float vr = rel_speed(s, a, b);
+            a = vs.slot(c.i + ia);
+            b = vs.slot(c.i + ib);
+            const float vr = rel_speed_v(vs.vel(a), vs.vel(b));
             sv = (double)vr * cp.sigma;
             acc = c.pc * vr > u;
         }
@@ -471,8 +520,8 @@ __device__ __forceinline__ int cell_trials_wave(const CollideParams &cp, const C
         for (int o = kWave / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, kWave));
         if (m > smax) smax = m;
         if (f < kWave) {
-            if (lane == f) scatter_pair(cp, stream, c.cell, step, tr, s, a, b);
-            __threadfence();  // the next round reads the updated velocities
+            if (lane == f) vs.scatter(cp, stream, c.cell, step, tr, a, b);
+            vs.publish();   // the next round reads the updated velocities
             ++count;
             tr0 += f + 1;
         } else {
@@ -483,20 +532,22 @@ __device__ __forceinline__ int cell_trials_wave(const CollideParams &cp, const C
 }
 
 constexpr int kCollItems = 8;  // slots per lane scanned for cell heads
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;   // no slot there (keys are below 2^31)
 constexpr int kWaveTrialsDefault = 16;
 
 // Trials of one cell (serial by one lane, or by the whole wave when heavy);
-// sigma v_r,max is stored when it grew.  Returns the accepted collisions.
+// sigma v_r,max is stored when it grew (unless `dry`).  Returns the accepted
+// collisions.
+template <class V>
 __device__ __forceinline__ int run_cell(const CollideParams &cp, const CellJob &c, uint32_t stream,
-                                        uint32_t step, const uint32_t *__restrict__ perm,
-                                        const FusedPtrs &s, float *__restrict__ sigvrmax,
-                                        bool wave, const Trial0 &t0)
+                                        uint32_t step, const V &vs, float *__restrict__ sigvrmax,
+                                        bool wave, const Trial0 &t0, bool dry = false)
 {
     double smax = c.smax;
-    const int cnt = wave ? cell_trials_wave(cp, c, stream, step, perm, s, smax)
-                         : cell_trials_serial(cp, c, stream, step, perm, s, smax, t0);
+    const int cnt = wave ? cell_trials_wave(cp, c, stream, step, vs, smax)
+                         : cell_trials_serial(cp, c, stream, step, vs, smax, t0);
     const bool writer = !wave || (threadIdx.x & (kWave - 1)) == 0;
-    if (writer && smax != c.smax) sigvrmax[c.k] = (float)smax;
+    if (writer && smax != c.smax && !dry) sigvrmax[c.k] = (float)smax;
     return cnt;
 }
 
@@ -523,11 +574,11 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     __shared__ short heavy[kFBlock / kWave][kChunk];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int chunk0 = (blockIdx.x * (kFBlock / kWave) + w) * kChunk;
+    const VelGlobal vg = { s.r, perm };
     const uint64_t lt = lanemask_lt();
     // the chunk's keys, the one before it and the 64 after it (for the end
     // of its last cell) are loaded with the alive count, the step and the
     // trap table; kNoKey marks slots past the alive atoms
-    constexpr uint32_t kNoKey = 0xFFFFFFFFu;   // keys are below 2^31
     uint32_t kk[kCollItems];
 #pragma unroll
     for (int it = 0; it < kCollItems; ++it) {
@@ -623,7 +674,7 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
             const uint32_t k = hkey[w][h];
             auto setup = [&](const TrapDev &tr, int t) {
                 stream = tr.stream;
-                trial0_load(cp, stream, k - (uint32_t)tr.key_base, step, end - i, i, perm, s, t0);
+                trial0_load(cp, stream, k - (uint32_t)tr.key_base, step, end - i, i, vg, t0);
                 ok = cell_setup(tr, t, k, cp, (double)sm, i, end, c);
             };
             if (t_lo == t_hi) {
@@ -638,7 +689,7 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
         if (is_heavy) heavy[w][nheavy + __popcll(hb & lt)] = (short)mi;
         nheavy += __popcll(hb);
         if (ok && !is_heavy) {
-            const int cnt = run_cell(cp, c, stream, step, perm, s, sigvrmax, false, t0);
+            const int cnt = run_cell(cp, c, stream, step, vg, sigvrmax, false, t0);
             if (t_lo == t_hi) wave_cnt += (unsigned long long)cnt;
             else if (cnt) atomicAdd(&coll_accum[c.t], (unsigned long long)cnt);
         }
@@ -652,7 +703,7 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
         CellJob c;
         (void)cell_setup(traps[t], t, k, cp, (double)sigvrmax[k], i, heads[w][h + 1], c);
         const Trial0 none = { 0u, 0u, 0.0f, 0.0f };
-        const int cnt = run_cell(cp, c, traps[t].stream, step, perm, s, sigvrmax, true, none);
+        const int cnt = run_cell(cp, c, traps[t].stream, step, vg, sigvrmax, true, none);
         if (lane == 0) {
             if (t_lo == t_hi) wave_cnt += (unsigned long long)cnt;
             else if (cnt) atomicAdd(&coll_accum[t], (unsigned long long)cnt);
@@ -792,22 +843,183 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
     key_out[u] = kout;   // cached: the re-sort reads the keys next
 }
 
+// Arguments of the collisions inside the fused step kernel (COLL).
+// `dry`: sigma v_r,max and the collision counts are not written (autotuner
+// trial launches).
+struct CollArgs {
+    CollideParams cp;
+    float *sigvrmax;
+    unsigned long long *coll_accum;
+    int wave_trials;
+    int dry;
+};
+
+// The fused step runs the collisions wave by wave: a wave holds W = 64 APT
+// consecutive sorted slots (its "span") and, next to them, kExt slots of the
+// neighbouring spans on each side.  A cell across a span boundary with at
+// most kExt slots on either side of it is run by both waves (the same
+// trials, so the same velocities; the wave holding its first slot stores
+// sigma v_r,max and counts its collisions).  Larger cells across a boundary
+// ("oversized") are run before the step kernel by k_collide_edges.
+constexpr int kExt = 16;
+
+// Oversized: the cell across boundary B (slots B - 1 and B hold key k) has
+// more than kExt slots on one side of it.  kb = key of slot B - kExt - 1
+// (kNoKey below 0), ka = key of slot B + kExt (kNoKey from n on).
+__device__ __forceinline__ bool oversized(uint32_t k, uint32_t kb, uint32_t ka)
+{
+    return kb == k || ka == k;
+}
+
+// LDS of one wave of the fused step: keys of the slots ws - kExt - 1 ..
+// ws + W + kExt (index slot - (ws - kExt - 1); kNoKey outside 0 .. n - 1),
+// velocities of the slots ws - kExt .. ws + W + kExt - 1 (index slot - (ws -
+// kExt)), sigma v_r,max of the span's slots, the cell list.
+template <int APT>
+struct WaveColl {
+    static constexpr int W = kWave * APT;
+    f4 v[W + 2 * kExt];
+    f2 w[W + 2 * kExt];
+    uint32_t k[W + 2 * kExt + 2];
+    float sm[W];
+    short mh[W / 2 + 1], me[W / 2 + 1], hv[W / 2 + 1];
+};
+
+// The collisions of one wave's span [ws, ws + len) (len <= W).  pre_h /
+// pre_e, post_h / post_e: the cells across its first and last boundary that
+// it runs (-1: none).  Each lane runs one cell (those with more than
+// wave_trials trials are run by the whole wave); the same trial sequence per
+// cell as k_collide_fused, so the velocities are bitwise the same.
+template <int APT>
+__device__ __forceinline__ void wave_collide(const TrapDev *__restrict__ traps, const int *ts, int T,
+                                             bool one, int t_first, const CollArgs &ca, uint32_t step,
+                                             int ws, int len, WaveColl<APT> &L, int pre_h, int pre_e,
+                                             int post_h)
+{
+    constexpr int W = WaveColl<APT>::W;
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t lt = lanemask_lt();
+    const int xb = ws - kExt - 1, vb = ws - kExt;
+    const int le = ws + len;
+    const VelLds vl = { L.v, L.w, vb };
+    // heads of the cells of >= 2 atoms in the span that this wave runs: those
+    // that end inside it, and the one across its last boundary when it is
+    // not oversized (at most 32 per 64 slots)
+    int nm = 0;
+#pragma unroll
+    for (int k = 0; k < APT; ++k) {
+        const int l = k * kWave + lane;
+        bool mult = false;
+        int e = 0;
+        if (l < len) {
+            const int sl = ws + l;
+            const uint32_t kk = L.k[sl - xb];
+            if (L.k[sl - 1 - xb] != kk && L.k[sl + 1 - xb] == kk) {
+                int lo = sl + 1, hi = le + kExt;   // first later slot with another key
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (L.k[mid - xb] == kk) lo = mid + 1; else hi = mid;
+                }
+                e = lo;
+                mult = e <= le || sl == post_h;
+            }
+        }
+        const uint64_t bm = __ballot(mult);
+        if (mult) {
+            const int p = nm + __popcll(bm & lt);
+            L.mh[p] = (short)(ws + l - vb);
+            L.me[p] = (short)(e - vb);
+        }
+        nm += __popcll(bm);
+    }
+    if (pre_h >= 0) {   // the cell across the first boundary
+        if (lane == 0) {
+            L.mh[nm] = (short)(pre_h - vb);
+            L.me[nm] = (short)(pre_e - vb);
+        }
+        ++nm;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    int nheavy = 0;
+    unsigned long long wave_cnt = 0;   // accepted collisions of owned cells (one-trap blocks)
+    for (int m0 = 0; m0 < nm; m0 += kWave) {
+        const int mi = m0 + lane;
+        CellJob c;
+        Trial0 t0;
+        bool ok = false, own = false;
+        uint32_t stream = 0;
+        if (mi < nm) {
+            const int h = vb + L.mh[mi], e = vb + L.me[mi];
+            own = h >= ws;
+            const uint32_t k = L.k[h - xb];
+            const int t = one ? t_first : last_le(ts, T, max(h, ws));
+            const TrapDev &tr = traps[t];
+            stream = tr.stream;
+            const float sv = own ? L.sm[h - ws] : ca.sigvrmax[k];
+            trial0_load(ca.cp, stream, k - (uint32_t)tr.key_base, step, e - h, h, vl, t0);
+            ok = cell_setup(tr, t, k, ca.cp, (double)sv, h, e, c);
+        }
+        const bool is_heavy = ok && c.ncol > ca.wave_trials;
+        const uint64_t hb = __ballot(is_heavy);
+        if (is_heavy) L.hv[nheavy + __popcll(hb & lt)] = (short)mi;
+        nheavy += __popcll(hb);
+        if (ok && !is_heavy) {
+            const int cnt = run_cell(ca.cp, c, stream, step, vl, ca.sigvrmax, false, t0, ca.dry != 0 || !own);
+            if (own) {
+                if (one) wave_cnt += (unsigned long long)cnt;
+                else if (cnt && !ca.dry) atomicAdd(&ca.coll_accum[c.t], (unsigned long long)cnt);
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int q = 0; q < nheavy; ++q) {
+        const int mi = L.hv[q];
+        const int h = vb + L.mh[mi], e = vb + L.me[mi];
+        const bool own = h >= ws;
+        const uint32_t k = L.k[h - xb];
+        const int t = one ? t_first : last_le(ts, T, max(h, ws));
+        CellJob c;
+        (void)cell_setup(traps[t], t, k, ca.cp, (double)(own ? L.sm[h - ws] : ca.sigvrmax[k]), h, e, c);
+        const Trial0 none = { 0u, 0u, 0.0f, 0.0f };
+        const int cnt = run_cell(ca.cp, c, traps[t].stream, step, vl, ca.sigvrmax, true, none,
+                                 ca.dry != 0 || !own);
+        if (lane == 0 && own) {
+            if (one) wave_cnt += (unsigned long long)cnt;
+            else if (cnt && !ca.dry) atomicAdd(&ca.coll_accum[t], (unsigned long long)cnt);
+        }
+    }
+    if (one) {
+        wave_cnt = wave_sum(wave_cnt);
+        if (lane == 0 && wave_cnt && !ca.dry) atomicAdd(&ca.coll_accum[t_first], wave_cnt);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    (void)W;
+}
+
 // A block of BS threads moves APT * BS consecutive slots, APT per thread: all
 // of a thread's atoms are loaded before the first is updated, so their loads
-// are in flight together.
+// are in flight together.  COLL (the fused step): between the loads and the
+// updates, the block runs the collisions of its cells on an LDS copy of the
+// velocities (block_collide).
 // Without spin, two atoms per thread fit 80 registers: six waves per SIMD.
-template <int BS, int APT, bool SPIN>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(APT == 2 && !SPIN ? 6 : 1)))
+template <int BS, int APT, bool SPIN, bool COLL>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(APT == 2 && !SPIN && !COLL ? 6 : 1)))
 void k_move_fused(
     const TrapDev *__restrict__ traps, MoveParams mp, FusedPtrs A, FusedPtrs B,
     const uint32_t *__restrict__ perm, const int *__restrict__ trap_start,
     const int *__restrict__ n_dev, uint32_t *__restrict__ key_out, float *__restrict__ evap_pos,
     float *__restrict__ evap_vel, int *__restrict__ rec_trap, double *__restrict__ rec_sum,
-    int rec_stride, double *__restrict__ acc_over, const long long *__restrict__ step_dev)
+    int rec_stride, double *__restrict__ acc_over, const long long *__restrict__ step_dev,
+    CollArgs ca)
 {
     extern __shared__ int ts[];
     __shared__ double red[(BS / kWave) * 4];
     __shared__ uint32_t wcnt[BS / kWave][2];
+    // (COLL) LDS of each wave's collisions
+    __shared__ WaveColl<APT> wcl[COLL ? BS / kWave : 1];
     const int T = mp.n_traps;
     load_table(ts, trap_start, T + 1);
     const int n = *n_dev;
@@ -832,7 +1044,159 @@ void k_move_fused(
         lost_e[k] = 0.0;
         lost[k] = moved[k] = false;
     }
-    if (t_first == t_last) {
+    if constexpr (COLL) {
+        // The fused step, wave by wave (no block barrier until the sums): the
+        // wave's span is slots ws .. ws + W - 1, a lane's atoms ws + 64 k +
+        // lane.  All loads are issued at once: each atom's slot in the
+        // previous order, its sorted key and sigma v_r,max of that cell, its
+        // three record planes, label and spin; and the keys and velocities of
+        // kExt slots on each side.  The velocity planes go to the wave's LDS,
+        // it runs the collisions of its cells there, and the atoms take their
+        // velocities back.  Positions, remainders and labels stay in
+        // registers.
+        constexpr int W = kWave * APT;
+        WaveColl<APT> &L = wcl[COLL ? threadIdx.x / kWave : 0];
+        const int lane = threadIdx.x & (kWave - 1);
+        const int ws = b0 + (int)(threadIdx.x / kWave) * W;
+        const int len = min(n - ws, W);   // <= 0: no atoms
+        const int xb = ws - kExt - 1, vb = ws - kExt;
+        const bool one = t_first == t_last;
+        uint32_t jj[APT];
+        f4 p0[APT];
+        float rpy[APT], rpz[APT];
+#pragma unroll
+        for (int k = 0; k < APT; ++k) {
+            const int i = ws + k * kWave + lane;
+            at[k].t = -1;
+            jj[k] = 0;
+            if (i < n) {
+                int t = t_first;
+                if (!one) while (t < t_last && ts[t + 1] <= i) ++t;
+                at[k].t = t;
+                jj[k] = perm ? ld_stream_at(perm, (uint32_t)i) : (uint32_t)i;
+                at[k].kold = ld_at(mp.key_prev, (uint32_t)i);
+            }
+        }
+        // neighbours: lanes 0 .. kExt load the kExt + 1 keys before the span
+        // (and the kExt velocities), lanes 32 .. 32 + kExt those after it
+        {
+            const bool pre = lane <= kExt, post = lane >= 32 && lane <= 32 + kExt;
+            const int sl = pre ? xb + lane : ws + W + (lane - 32);
+            uint32_t kn = kNoKey;
+            f4 v1 = f4{ 0.0f, 0.0f, 0.0f, 0.0f }, v2 = v1;
+            const bool in = (pre || post) && sl >= 0 && sl < n;
+            if (in) {
+                kn = ld_at(mp.key_prev, (uint32_t)sl);
+                const bool vel = pre ? lane >= 1 : lane < 32 + kExt;   // slots with a velocity entry
+                if (vel) {
+                    const uint32_t j = perm ? ld_stream_at(perm, (uint32_t)sl) : (uint32_t)sl;
+                    const f4 *q = A.r + rec_index(j);
+                    v1 = q[kRecTile];
+                    v2 = q[2 * kRecTile];
+                }
+            }
+            if (pre) {
+                L.k[lane] = kn;
+                if (lane >= 1) { L.v[lane - 1] = v1; L.w[lane - 1] = f2{ v2.z, v2.w }; }
+            } else if (post) {
+                L.k[W + kExt + 1 + (lane - 32)] = kn;
+                if (lane < 32 + kExt) {
+                    L.v[W + kExt + (lane - 32)] = v1;
+                    L.w[W + kExt + (lane - 32)] = f2{ v2.z, v2.w };
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < APT; ++k) {
+            const int l = k * kWave + lane;
+            L.k[kExt + 1 + l] = at[k].t >= 0 ? at[k].kold : kNoKey;
+            if (at[k].t >= 0) {
+                const f4 *q = A.r + rec_index(jj[k]);
+                L.sm[l] = ca.sigvrmax[at[k].kold];
+                const f4 v1 = __builtin_nontemporal_load(q + kRecTile);
+                const f4 v2 = __builtin_nontemporal_load(q + 2 * kRecTile);
+                p0[k] = __builtin_nontemporal_load(q);
+                at[k].id = ld_stream_at(A.id, jj[k]);
+                at[k].sx = at[k].sy = at[k].sz = 0.0f;
+                if (SPIN) {
+                    at[k].sx = ld_at(A.s[0], jj[k]);
+                    at[k].sy = ld_at(A.s[1], jj[k]);
+                    at[k].sz = ld_at(A.s[2], jj[k]);
+                }
+                L.v[kExt + l] = v1;
+                L.w[kExt + l] = f2{ v2.z, v2.w };
+                rpy[k] = v2.x;
+                rpz[k] = v2.y;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // the cells across the span's first and last boundary that it runs
+        int pre_h = -1, pre_e = -1, post_h = -1;
+        if (len > 0 && ws > 0) {
+            const uint32_t k0 = L.k[ws - xb];
+            if (L.k[ws - 1 - xb] == k0 && !oversized(k0, L.k[ws - kExt - 1 - xb], L.k[ws + kExt - xb])) {
+                int lo = ws - kExt, hi = ws - 1;   // first slot with k0
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (L.k[mid - xb] == k0) hi = mid; else lo = mid + 1;
+                }
+                pre_h = lo;
+                lo = ws + 1; hi = ws + kExt;       // first later slot with another key
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (L.k[mid - xb] == k0) lo = mid + 1; else hi = mid;
+                }
+                pre_e = lo;
+            }
+        }
+        const int le = ws + len;
+        if (len == W && le < n) {
+            const uint32_t k1 = L.k[le - 1 - xb];
+            if (L.k[le - xb] == k1 && !oversized(k1, L.k[le - kExt - 1 - xb], L.k[le + kExt - xb])) {
+                int lo = le - kExt, hi = le - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (L.k[mid - xb] == k1) hi = mid; else lo = mid + 1;
+                }
+                post_h = lo;
+            }
+        }
+        if (len > 0)
+            wave_collide<APT>(traps, ts, T, one, t_first, ca, (uint32_t)step, ws, len, L, pre_h, pre_e,
+                              post_h);
+#pragma unroll
+        for (int k = 0; k < APT; ++k) {
+            const int l = k * kWave + lane;
+            if (at[k].t >= 0) {
+                const f4 v = L.v[kExt + l];
+                const f2 w = L.w[kExt + l];
+                at[k].f.px = p0[k].x; at[k].f.py = p0[k].y; at[k].f.pz = p0[k].z; at[k].f.rpx = p0[k].w;
+                at[k].f.vx = v.x; at[k].f.vy = v.y; at[k].f.vz = v.z; at[k].f.rvx = v.w;
+                at[k].f.rpy = rpy[k]; at[k].f.rpz = rpz[k];
+                at[k].f.rvy = w.x; at[k].f.rvz = w.y;
+            }
+        }
+        (void)vb;
+        if (one) {
+            const TrapDev &tr = traps[t_first];
+#pragma unroll
+            for (int k = 0; k < APT; ++k) {
+                const int i = ws + k * kWave + lane;
+                if (at[k].t >= 0)
+                    move_update<SPIN>(mp, tr, at[k], step, i, B, key_out, evap_pos, evap_vel, vals[k],
+                                      lost_e[k], lost[k], moved[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < APT; ++k) {
+                const int i = ws + k * kWave + lane;
+                if (at[k].t >= 0)
+                    move_update<SPIN>(mp, traps[at[k].t], at[k], step, i, B, key_out, evap_pos, evap_vel,
+                                      vals[k], lost_e[k], lost[k], moved[k]);
+            }
+        }
+    } else if (t_first == t_last) {
         // one trap: its parameters are wave-uniform (scalar loads and branches)
         const TrapDev &tr = traps[t_first];
 #pragma unroll
@@ -888,7 +1252,9 @@ void k_move_fused(
             cm += (uint32_t)__popcll(bm);
             cd += (uint32_t)__popcll(bd);
             if ((threadIdx.x & (kWave - 1)) == 0) {
-                const int wd = (b0 + k * BS) / kWave + (int)threadIdx.x / kWave;
+                // the 64-slot word of this wave's k-th atoms
+                const int wd = COLL ? b0 / kWave + (int)(threadIdx.x / kWave) * APT + k
+                                    : (b0 + k * BS) / kWave + (int)threadIdx.x / kWave;
                 mp.mask_moved[wd] = bm;
                 mp.mask_dead[wd] = bd;
                 if (at[k].t >= 0) {   // slot 64 wd is below n
@@ -1067,7 +1433,9 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
     o.ep = f[kAccEp];
     o.v2[0] = f[kAccV2x]; o.v2[1] = f[kAccV2y]; o.v2[2] = f[kAccV2z];
     o.e_evap = f[kAccEevap];
-    o.n_alive = (long long)(s1 - s0);
+    // survivors of this step (the removed atoms are still in the slot range
+    // the move covered; the next sort drops them)
+    o.n_alive = (long long)(s1 - s0) - (long long)f[kAccNevap];
     o.n_evap = (long long)f[kAccNevap];
     o.collisions = (long long)coll;
     o.step = step;
@@ -1080,6 +1448,103 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
             *n_next = nn;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// On-demand observables of the current FP32 state (dsmc_observe,
+// dsmc_calculate_temp).  The slots hold the traps in order: each trap's alive
+// atoms are contiguous except for removed atoms (dead key) that a move left
+// in place.  The trap of a slot comes from its key.  First pass: the first
+// and last alive slot of every trap (integer min / max, the same in any
+// order of execution).  Second pass: one block per trap sums its alive slots
+// in a fixed order, so the sums are reproducible run to run.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kFBlock) void k_obs_bounds(const int *__restrict__ key_base, int T,
+                                                        const uint32_t *__restrict__ key,
+                                                        const int *__restrict__ n_dev,
+                                                        uint32_t dead_key, int *__restrict__ first,
+                                                        int *__restrict__ last)
+{
+    extern __shared__ int kb[];
+    load_table(kb, key_base, T + 1);
+    const int i = blockIdx.x * kFBlock + threadIdx.x;
+    const int n = *n_dev;
+    const uint32_t k = i < n ? key[i] : dead_key;
+    const bool live = k != dead_key;
+    const int t = live ? last_le(kb, T, (int)k) : -1;
+    const uint64_t lm = __ballot(live);
+    if (!lm) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int l0 = __ffsll((unsigned long long)lm) - 1, l1 = 63 - __clzll((long long)lm);
+    const int t0 = __shfl(t, l0, kWave), t1 = __shfl(t, l1, kWave);
+    if (t0 == t1) {   // the wave's alive slots lie in one trap
+        if (lane == 0) {
+            const int i0 = i - lane;
+            atomicMin(&first[t0], i0 + l0);
+            atomicMax(&last[t0], i0 + l1);
+        }
+    } else if (live) {
+        atomicMin(&first[t], i);
+        atomicMax(&last[t], i);
+    }
+}
+
+constexpr int kObsBlock = 1024;
+
+__global__ __launch_bounds__(kObsBlock) void k_obs_sums(const TrapDev *__restrict__ traps,
+                                                        double mass, FusedPtrs s,
+                                                        const uint32_t *__restrict__ key,
+                                                        uint32_t dead_key,
+                                                        const int *__restrict__ first,
+                                                        const int *__restrict__ last,
+                                                        double *__restrict__ acc)
+{
+    __shared__ double red[(kObsBlock / kWave) * 6];
+    const int t = blockIdx.x;
+    const TrapDev &tr = traps[t];
+    const int a = first[t], b = last[t];
+    double v[6] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };   // v2x, v2y, v2z, U, n, (unused)
+    for (int i = a + (int)threadIdx.x; i <= b; i += kObsBlock) {
+        if (key[i] == dead_key) continue;
+        FState f;
+        rec_load(s.r, (uint32_t)i, f);
+        const double X = (double)f.px + (double)f.rpx;
+        const double Y = (double)f.py + (double)f.rpy;
+        const double Z = (double)f.pz + (double)f.rpz;
+        const double VX = (double)f.vx + (double)f.rvx;
+        const double VY = (double)f.vy + (double)f.rvy;
+        const double VZ = (double)f.vz + (double)f.rvz;
+        v[0] += VX * VX;
+        v[1] += VY * VY;
+        v[2] += VZ * VZ;
+        v[3] += potential_energy(tr, X, Y, Z);
+        v[4] += 1.0;
+    }
+    block_sum<kObsBlock, 6>(v, red);
+    if (threadIdx.x == 0) {
+        double *o = acc + (size_t)t * kAccK;
+        o[kAccEk] = 0.5 * mass * (v[0] + v[1] + v[2]);
+        o[kAccEp] = v[3];
+        o[kAccV2x] = v[0]; o[kAccV2y] = v[1]; o[kAccV2z] = v[2];
+        o[kAccEevap] = 0.0;
+        o[kAccNevap] = 0.0;
+        o[kAccN] = v[4];
+    }
+}
+
+void launch_observe_fused(const TrapDev *traps, const int *key_base, int n_traps, double mass,
+                          const FusedPtrs &s, const uint32_t *key, const int *n_dev, int n_cap,
+                          uint32_t dead_key, int *first, int *last, double *acc, hipStream_t st)
+{
+    fill_u32((uint32_t *)first, 0x7FFFFFFFu, n_traps, st, nullptr);
+    fill_u32((uint32_t *)last, 0xFFFFFFFFu, n_traps, st, nullptr);
+    hipLaunchKernelGGL(k_obs_bounds, dim3(fgrid(n_cap, kFBlock)), dim3(kFBlock),
+                       sizeof(int) * (size_t)(n_traps + 1), st, key_base, n_traps, key, n_dev,
+                       dead_key, first, last);
+    DSMC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_obs_sums, dim3(n_traps), dim3(kObsBlock), 0, st, traps, mass, s, key,
+                       dead_key, first, last, acc);
+    DSMC_HIP(hipGetLastError());
 }
 
 // dst[i] = src[map[i]] for every FP32 state array.
@@ -1103,7 +1568,6 @@ __global__ __launch_bounds__(kFBlock) void k_gather_fused(FusedPtrs dst, FusedPt
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
-static int fgrid(long long n, int bs) { return div_up(n > 0 ? n : 1, bs); }
 
 void launch_init_fused(const TrapDev *traps, const int *trap_start, int n_traps,
                        const InitParams &ip, const FusedPtrs &s, int n, hipStream_t st,
@@ -1120,12 +1584,12 @@ void launch_init_fused(const TrapDev *traps, const int *trap_start, int n_traps,
 
 void launch_index_fused(const TrapDev *traps, const int *trap_start, int n_traps,
                         const FusedPtrs &s, uint32_t *key, const int *n_dev, int n_cap,
-                        hipStream_t st, Profiler *prof)
+                        uint32_t dead_key, bool keep_dead, hipStream_t st, Profiler *prof)
 {
     const size_t lds = sizeof(int) * (size_t)(n_traps + 1);
     if (prof && prof->enabled) prof->begin(st, "index_fused", (double)n_cap * 28.0);
     hipLaunchKernelGGL(k_index_fused, dim3(fgrid(n_cap, kFBlock)), dim3(kFBlock), lds, st, traps,
-                       trap_start, n_traps, s, key, n_dev);
+                       trap_start, n_traps, s, key, n_dev, dead_key, keep_dead ? 1 : 0);
     DSMC_HIP(hipGetLastError());
     if (prof && prof->enabled) prof->end(st);
 }
@@ -1177,40 +1641,62 @@ double move_fused_bytes_per_atom(int spin)
     return 4.0 + state + state + 4.0;
 }
 
-template <int BS, int APT>
+template <int BS, int APT, bool COLL>
 static void move_fused_bs(const TrapDev *traps, const MoveParams &mp, const FusedPtrs &src,
                           const FusedPtrs &dst, const uint32_t *perm, const int *trap_start,
                           const int *n_dev, uint32_t *key_out, float *evap_pos, float *evap_vel,
-                          const ObsRecords &rec, const long long *step_dev, int n_cap,
-                          hipStream_t st)
+                          const ObsRecords &rec, const long long *step_dev, const CollArgs &ca,
+                          int n_cap, hipStream_t st)
 {
     const size_t lds = sizeof(int) * (size_t)(mp.n_traps + 1);
     if (mp.spin)
-        hipLaunchKernelGGL((k_move_fused<BS, APT, true>), dim3(fgrid(n_cap, BS * APT)), dim3(BS),
+        hipLaunchKernelGGL((k_move_fused<BS, APT, true, COLL>), dim3(fgrid(n_cap, BS * APT)), dim3(BS),
                            lds, st, traps, mp, src, dst, perm, trap_start, n_dev, key_out,
                            evap_pos, evap_vel, rec.trap, rec.sum, rec.stride, rec.acc_over,
-                           step_dev);
+                           step_dev, ca);
     else
-        hipLaunchKernelGGL((k_move_fused<BS, APT, false>), dim3(fgrid(n_cap, BS * APT)), dim3(BS),
+        hipLaunchKernelGGL((k_move_fused<BS, APT, false, COLL>), dim3(fgrid(n_cap, BS * APT)), dim3(BS),
                            lds, st, traps, mp, src, dst, perm, trap_start, n_dev, key_out,
                            evap_pos, evap_vel, rec.trap, rec.sum, rec.stride, rec.acc_over,
-                           step_dev);
+                           step_dev, ca);
 }
 
 int obs_record_stride(int n_cap) { return fgrid(n_cap, 64) + 1; }
+
+static int wave_trials_setting()
+{
+    // DSMC_COLLIDE_WAVE_TRIALS: trial count above which a cell is run by a
+    // whole wave (results are the same either way)
+    const char *env = std::getenv("DSMC_COLLIDE_WAVE_TRIALS");
+    return env ? std::atoi(env) : kWaveTrialsDefault;
+}
 
 void launch_move_fused(int block, int apt, const TrapDev *traps, const MoveParams &mp,
                        const FusedPtrs &src, const FusedPtrs &dst, const uint32_t *perm,
                        const int *trap_start, const int *n_dev, uint32_t *key_out,
                        float *evap_pos, float *evap_vel, const ObsRecords &rec,
-                       const long long *step_dev, int n_cap, hipStream_t st, Profiler *prof)
+                       const long long *step_dev, int n_cap, hipStream_t st, Profiler *prof,
+                       const StepCollide *coll)
 {
+    CollArgs ca;
+    std::memset(&ca, 0, sizeof ca);
+    if (coll) {
+        ca.cp = coll->cp;
+        ca.sigvrmax = coll->sigvrmax;
+        ca.coll_accum = coll->coll_accum;
+        ca.wave_trials = wave_trials_setting();
+        ca.dry = coll->dry;
+    }
     if (prof && prof->enabled)
-        prof->begin(st, "move_fused", (double)n_cap * move_fused_bytes_per_atom(mp.spin));
+        prof->begin(st, coll ? "step_fused" : "move_fused", (double)n_cap * move_fused_bytes_per_atom(mp.spin));
 #define DSMC_MOVE_CASE(BSV, APTV)                                                               \
     if (block == BSV && apt == APTV) {                                                          \
-        move_fused_bs<BSV, APTV>(traps, mp, src, dst, perm, trap_start, n_dev, key_out,         \
-                                 evap_pos, evap_vel, rec, step_dev, n_cap, st);                 \
+        if (coll)                                                                               \
+            move_fused_bs<BSV, APTV, true>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, \
+                                           evap_pos, evap_vel, rec, step_dev, ca, n_cap, st);   \
+        else                                                                                    \
+            move_fused_bs<BSV, APTV, false>(traps, mp, src, dst, perm, trap_start, n_dev, key_out,\
+                                            evap_pos, evap_vel, rec, step_dev, ca, n_cap, st);  \
     } else
     DSMC_MOVE_CASE(64, 1)
     DSMC_MOVE_CASE(128, 1)
@@ -1221,10 +1707,111 @@ void launch_move_fused(int block, int apt, const TrapDev *traps, const MoveParam
     DSMC_MOVE_CASE(256, 2)
     DSMC_MOVE_CASE(512, 2)
     {
-        move_fused_bs<256, 1>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
-                              evap_vel, rec, step_dev, n_cap, st);
+        if (coll)
+            move_fused_bs<256, 1, true>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
+                                        evap_vel, rec, step_dev, ca, n_cap, st);
+        else
+            move_fused_bs<256, 1, false>(traps, mp, src, dst, perm, trap_start, n_dev, key_out, evap_pos,
+                                         evap_vel, rec, step_dev, ca, n_cap, st);
     }
 #undef DSMC_MOVE_CASE
+    DSMC_HIP(hipGetLastError());
+    if (prof && prof->enabled) prof->end(st);
+}
+
+// ---------------------------------------------------------------------------
+// Edge pass of the fused step: the oversized cells across the boundaries
+// between blocks of the step kernel (block = move_block sorted slots), i.e.
+// cells that hold slots B - 1 and B for some B = q * move_block < n and have
+// more than kExt slots on one side of B.  One lane per boundary checks it
+// (four keys); each oversized cell is then run by the whole wave at the
+// first boundary it crosses (its first slot in [B - move_block, B)), on the
+// state in HBM through the permutation, as k_collide_fused does, before the
+// step kernel gathers it.  Same trials as k_collide_fused for the same cell.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kFBlock) void k_collide_edges(
+    const TrapDev *__restrict__ traps, const int *__restrict__ key_base, int T, CollideParams cp,
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ perm,
+    const int *__restrict__ n_dev, FusedPtrs s, float *__restrict__ sigvrmax,
+    unsigned long long *__restrict__ coll_accum, const long long *__restrict__ step_dev,
+    int move_block, int wave_trials)
+{
+    extern __shared__ int kb[];
+    load_table(kb, key_base, T + 1);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int n = *n_dev;
+    const long long q = (long long)blockIdx.x * kFBlock + threadIdx.x + 1;
+    const long long B = q * move_block;
+    bool over = false;
+    uint32_t k = 0;
+    if (B < n) {
+        const int b = (int)B;
+        k = key[b];
+        const uint32_t kp = key[b - 1];
+        const uint32_t kbf = b - kExt - 1 >= 0 ? key[b - kExt - 1] : kNoKey;
+        const uint32_t kaf = b + kExt < n ? key[b + kExt] : kNoKey;
+        over = kp == k && oversized(k, kbf, kaf);
+        // run at the first boundary the cell crosses
+        const int lo = b - move_block;
+        if (over && lo > 0 && key[lo - 1] == k) over = false;
+    }
+    uint64_t todo = __ballot(over);
+    if (!todo) return;   // whole wave
+    const VelGlobal vg = { s.r, perm };
+    const uint32_t step = (uint32_t)*step_dev;
+    while (todo) {
+        const int src = __ffsll((unsigned long long)todo) - 1;
+        todo &= todo - 1;
+        const uint32_t kc = __shfl(k, src, kWave);
+        const int b = (int)(((long long)blockIdx.x * kFBlock + (threadIdx.x & ~(kWave - 1)) + src + 1) * move_block);
+        const int t = last_le(kb, T, (int)kc);
+        const TrapDev &tr = traps[t];
+        if ((int)(kc - (uint32_t)tr.key_base) >= tr.n_cells) continue;   // overflow cell: no collisions
+        // first slot: probed 64 at a time below b (it lies in [b - move_block, b))
+        int head = b - move_block;
+        for (int j0 = b - kWave; j0 > b - move_block - kWave; j0 -= kWave) {
+            const int j = j0 + lane;
+            const uint64_t d = __ballot(j >= b - move_block && j < b && key[j] != kc);
+            if (d) { head = j0 + 64 - __clzll((long long)d); break; }
+        }
+        // end: the first slot after b with another key, probed 64 at a time
+        int end = n;
+        for (int j0 = b + 1; j0 < n; j0 += kWave) {
+            const int j = j0 + lane;
+            const uint64_t d = __ballot(j < n && key[j] != kc);
+            if (d) { end = j0 + __ffsll((unsigned long long)d) - 1; break; }
+        }
+        CellJob c;
+        if (!cell_setup(tr, t, kc, cp, (double)sigvrmax[kc], head, end, c)) continue;
+        int cnt;
+        if (c.ncol > wave_trials) {
+            const Trial0 none = { 0u, 0u, 0.0f, 0.0f };
+            cnt = run_cell(cp, c, tr.stream, step, vg, sigvrmax, true, none);
+        } else {
+            cnt = 0;
+            if (lane == 0) {
+                Trial0 t0;
+                trial0_load(cp, tr.stream, c.cell, step, c.n, c.i, vg, t0);
+                cnt = run_cell(cp, c, tr.stream, step, vg, sigvrmax, false, t0);
+            }
+            vg.publish();
+        }
+        if (lane == 0 && cnt) atomicAdd(&coll_accum[t], (unsigned long long)cnt);
+    }
+}
+
+void launch_collide_edges(const TrapDev *traps, const int *key_base, int n_traps,
+                          const CollideParams &cp, const uint32_t *key, const uint32_t *perm,
+                          const int *n_dev, const FusedPtrs &s, float *sigvrmax,
+                          unsigned long long *coll_accum, const long long *step_dev, int move_block,
+                          int n_cap, hipStream_t st, Profiler *prof)
+{
+    const int boundaries = std::max(1, fgrid(n_cap, move_block) - 1);
+    const size_t lds = sizeof(int) * (size_t)(n_traps + 1);
+    if (prof && prof->enabled) prof->begin(st, "collide_edges", 0.0);
+    hipLaunchKernelGGL(k_collide_edges, dim3(div_up(boundaries, kFBlock)), dim3(kFBlock), lds,
+                       st, traps, key_base, n_traps, cp, key, perm, n_dev, s, sigvrmax, coll_accum,
+                       step_dev, move_block, wave_trials_setting());
     DSMC_HIP(hipGetLastError());
     if (prof && prof->enabled) prof->end(st);
 }

@@ -131,7 +131,7 @@ void launch_init_fused(const TrapDev *traps, const int *trap_start, int n_traps,
                        Profiler *prof);
 void launch_index_fused(const TrapDev *traps, const int *trap_start, int n_traps,
                         const FusedPtrs &s, uint32_t *key, const int *n_dev, int n_cap,
-                        hipStream_t st, Profiler *prof);
+                        uint32_t dead_key, bool keep_dead, hipStream_t st, Profiler *prof);
 void launch_table_fused(const uint32_t *key, const int *key_base, const TableParams &tp,
                         const int *n_in, int *n_out, int *trap_start, int n_cap, hipStream_t st,
                         Profiler *prof);
@@ -153,14 +153,33 @@ struct ObsRecords {
     unsigned int *cnt; // finalize counters per trap                [T]
 };
 int  obs_record_stride(int n_cap);
+// Collisions inside the fused step kernel (see launch_move_fused).
+struct StepCollide {
+    CollideParams cp;
+    float *sigvrmax;
+    unsigned long long *coll_accum;
+    int dry;   // compute but write neither sigma v_r,max nor counts (autotuner trials)
+};
 // block: threads per block (64 .. 1024); apt: atoms per thread (1 or 2).  A
 // block moves block * apt consecutive slots and leaves one observables record
-// and one pair of re-sort counts.
+// and one pair of re-sort counts.  With coll (the fused step; needs the
+// sorted keys in mp.key_prev), each wave first runs the collisions of the
+// cells of its 64 * apt slots; the oversized cells across those spans'
+// boundaries must have been run by launch_collide_edges with move_block =
+// 64 * apt.
 void launch_move_fused(int block, int apt, const TrapDev *traps, const MoveParams &mp,
                        const FusedPtrs &src, const FusedPtrs &dst, const uint32_t *perm,
                        const int *trap_start, const int *n_dev, uint32_t *key_out,
                        float *evap_pos, float *evap_vel, const ObsRecords &rec,
-                       const long long *step_dev, int n_cap, hipStream_t st, Profiler *prof);
+                       const long long *step_dev, int n_cap, hipStream_t st, Profiler *prof,
+                       const StepCollide *coll = nullptr);
+// The collisions of the oversized cells across the boundaries between spans
+// of move_block sorted slots (the edge pass of the fused step).
+void launch_collide_edges(const TrapDev *traps, const int *key_base, int n_traps,
+                          const CollideParams &cp, const uint32_t *key, const uint32_t *perm,
+                          const int *n_dev, const FusedPtrs &s, float *sigvrmax,
+                          unsigned long long *coll_accum, const long long *step_dev, int move_block,
+                          int n_cap, hipStream_t st, Profiler *prof);
 // When n_next is set (the move followed a sort), the last block also writes
 // the next step's trap ranges (trap_start) and alive count (*n_next): each
 // trap loses its atoms removed in this step, which the next sort puts last.
@@ -176,6 +195,12 @@ void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, double
                          long long *step_next, dsmc_observables *obs_log, int obs_cap,
                          int *n_next, const ResortPrepArgs &prep, hipStream_t st,
                          Profiler *prof);
+// Observables of the current state per trap into acc (kAccK doubles per
+// trap; kAccN = alive atoms), skipping slots with the dead key; a fixed-order
+// reduction.  first / last: n_traps ints of scratch each.
+void launch_observe_fused(const TrapDev *traps, const int *key_base, int n_traps, double mass,
+                          const FusedPtrs &s, const uint32_t *key, const int *n_dev, int n_cap,
+                          uint32_t dead_key, int *first, int *last, double *acc, hipStream_t st);
 void launch_gather_fused(const FusedPtrs &dst, const FusedPtrs &src, const uint32_t *map,
                          const int *n_dev, int n_cap, int spin, hipStream_t st, Profiler *prof);
 double move_fused_bytes_per_atom(int spin);

@@ -862,22 +862,66 @@ void fill_f64(double *p, double v, long long n, hipStream_t stream, Profiler *pr
     DSMC_LAUNCH(prof, "fill", (double)n * 8.0, k_fill_f64, fill_grid(n), 256, stream, p, v, n);
 }
 
-// Streaming copy (16-byte vectors, grid-stride): the bandwidth reference used
-// to calibrate the HBM-traffic counters.
-__global__ __launch_bounds__(256) void k_stream_copy(const uint4 *__restrict__ src,
-                                                     uint4 *__restrict__ dst, long long n16)
+// Streaming copy for counter calibration: four 16-byte vectors per thread
+// and round, loads issued together, non-temporal loads and stores (each byte
+// is touched once), grid-stride over whole rounds.
+constexpr int kCopyUnroll = 4;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_stream_copy(const u32x4 *__restrict__ src,
+                                                     u32x4 *__restrict__ dst, long long n16)
 {
-    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n16; i += (long long)gridDim.x * 256)
-        dst[i] = src[i];
+    const long long stride = (long long)gridDim.x * 256 * kCopyUnroll;
+    for (long long i0 = (long long)blockIdx.x * 256 * kCopyUnroll + threadIdx.x; i0 < n16; i0 += stride) {
+        u32x4 v[kCopyUnroll];
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; ++u) {
+            const long long i = i0 + (long long)u * 256;
+            if (i < n16) v[u] = __builtin_nontemporal_load(src + i);
+        }
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; ++u) {
+            const long long i = i0 + (long long)u * 256;
+            if (i < n16) __builtin_nontemporal_store(v[u], dst + i);
+        }
+    }
 }
 
 void stream_copy(void *dst, const void *src, size_t bytes, hipStream_t stream)
 {
     const long long n16 = (long long)(bytes / 16);
     if (n16 <= 0) return;
-    const int grid = (int)std::min<long long>((n16 + 255) / 256, 256LL * 64);
-    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, stream, (const uint4 *)src,
-                       (uint4 *)dst, n16);
+    const int grid = (int)std::min<long long>((n16 + 256 * kCopyUnroll - 1) / (256 * kCopyUnroll), 8192LL);
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, stream, (const u32x4 *)src,
+                       (u32x4 *)dst, n16);
+    DSMC_HIP(hipGetLastError());
+}
+
+// Gather of the FP32 state's tiled 16-byte records (three planes of 64
+// records per 64-slot tile) through a slot map: record planes of slot i of
+// dst = those of slot map[i] of src.  The access pattern of the fused move's
+// state gather, for calibrating the traffic counters on it: 4 bytes of map
+// and 48 bytes of records read, 48 bytes written per slot.
+__global__ __launch_bounds__(256) void k_gather_copy(const u32x4 *__restrict__ src,
+                                                     u32x4 *__restrict__ dst,
+                                                     const uint32_t *__restrict__ map, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = __builtin_nontemporal_load(map + i);
+    const size_t qs = (size_t)(j / 64) * 192 + (j % 64), qd = (size_t)(i / 64) * 192 + (i % 64);
+    const u32x4 a = __builtin_nontemporal_load(src + qs), b = __builtin_nontemporal_load(src + qs + 64),
+                c = __builtin_nontemporal_load(src + qs + 128);
+    __builtin_nontemporal_store(a, dst + qd);
+    __builtin_nontemporal_store(b, dst + qd + 64);
+    __builtin_nontemporal_store(c, dst + qd + 128);
+}
+
+void gather_copy(void *dst, const void *src, const uint32_t *map, int n, hipStream_t stream)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_copy, dim3(div_up(n, 256)), dim3(256), 0, stream, (const u32x4 *)src,
+                       (u32x4 *)dst, map, n);
     DSMC_HIP(hipGetLastError());
 }
 

@@ -108,7 +108,8 @@ typedef struct dsmc_observables {
     double    ep;           /* sum U(r) (J)                                       */
     double    v2[3];        /* sum vx^2, vy^2, vz^2 (m^2/s^2)                     */
     double    e_evap;       /* energy of the particles removed in this step (J)   */
-    long long n_alive;      /* particles moved in this step                       */
+    long long n_alive;      /* particles alive after this step (moved and kept);  */
+                            /* n_alive + n_evap were moved in this step           */
     long long n_evap;       /* particles removed in this step                     */
     long long collisions;   /* accepted collisions in this step                   */
     long long step;         /* motion step index                                  */
@@ -130,7 +131,9 @@ typedef struct dsmc_info {
 
 /* Fields for dsmc_copy_out / dsmc_copy_in.  Per-atom fields are in slot order
  * of the current state buffer, double precision, AoS (x, y, z) triples, the
- * layout the reference copies to the host (main.cu:526-536). */
+ * layout the reference copies to the host (main.cu:526-536).  They hold the
+ * alive atoms only: in the FP32 form an atom removed by a move keeps its slot
+ * (with the dead key) until the next sort drops it, and reads skip it. */
 enum {
     DSMC_FIELD_POS = 0,        /* n x 3 double                                     */
     DSMC_FIELD_VEL,            /* n x 3 double                                     */
@@ -297,9 +300,18 @@ dsmc_status dsmc_philox(unsigned int *d_out, int n, const unsigned int ctr[4],
 
 /* Streaming device-to-device copy of `bytes` (a multiple of 16), launched
  * `repeats` times on the default stream; *ms = mean time per copy.  No
- * reference counterpart: the bandwidth reference against which the step's
- * kernels and the HBM-traffic counters are calibrated. */
+ * reference counterpart: a kernel of known bytes on which the HBM-traffic
+ * counters are calibrated. */
 dsmc_status dsmc_stream_copy(void *d_dst, const void *d_src, size_t bytes, int repeats, double *ms);
+/* Gather of n_slots slots of FP32-form state records (48 bytes a slot, laid
+ * out in tiles of 64 slots: three planes of 64 16-byte records) from d_src
+ * into d_dst, slot i taking slot d_map[i]; both buffers hold whole tiles.
+ * Launched `repeats` times on the default stream; *ms = mean time per launch.
+ * No reference counterpart: a kernel of known bytes (52 read, 48 written a
+ * slot) with the access pattern of the fused move's state gather, on which
+ * the HBM-traffic counters are calibrated. */
+dsmc_status dsmc_gather_copy(void *d_dst, const void *d_src, const unsigned int *d_map, int n_slots,
+                             int repeats, double *ms);
 
 /* Device memory helpers for hosts without their own allocator. */
 dsmc_status dsmc_device_alloc(void **d_ptr, size_t bytes);

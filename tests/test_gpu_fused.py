@@ -170,6 +170,46 @@ def test_incremental_sort_equals_full_sort(monkeypatch):
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("case", ["batched", "dense-64", "dense-256"])
+def test_fused_step_equals_split(case, monkeypatch):
+    """The fused step (DSMC_STEP_FUSED=1: collisions inside the move kernel on
+    an LDS copy of each wave's velocities and its neighbours', oversized cells
+    across span boundaries in an edge pass) against the collide and move
+    kernels launched one after the other: bitwise equal states, labels,
+    collision counts and removals.  "batched": three traps with evaporation,
+    graph replay and a block-size autotune half way; "dense": cells of tens to
+    hundreds of atoms (many cross span boundaries, many run wave-parallel),
+    with 64- and 256-thread blocks."""
+    if case == "batched":
+        run, steps = _batch_run(), (20, 21)
+    else:
+        trap = configs.quadrupole_trap(100000, 12, n_real=3e9)
+        bs = 64 if case == "dense-64" else 256
+        run = configs.Run(traps=[trap], precision="fp32", dt=1e-6, seed=5, block_size=bs,
+                          name="dense-cells")
+        steps = (10, 11)
+    out = {}
+    for split in ("1", "0"):
+        monkeypatch.setenv("DSMC_STEP_FUSED", "0" if split == "1" else "1")
+        print(f"fused vs split step: {case}, split={split}", flush=True)
+        with Simulation(run) as sim:
+            sim.init_distribution()
+            sim.step(steps[0])
+            if case == "batched":
+                sim.autotune()
+            sim.step(steps[1])
+            lg = sim.obs_log()
+            out[split] = [sim.copy_out(f) for f in (_abi.FIELD_POS, _abi.FIELD_VEL, _abi.FIELD_ATOM_ID)]
+            out[split] += [lg["collisions"].copy(), lg["n_evap"].copy(), lg["ek"].copy()]
+    assert int(out["1"][3].sum()) > 100
+    for a, b in zip(out["1"][:5], out["0"][:5]):
+        np.testing.assert_array_equal(a, b)
+    # the FP64 energy sums are per move block: the autotuner may choose
+    # another block size for the fused kernel, which changes only the order
+    # of the additions
+    np.testing.assert_allclose(out["1"][5], out["0"][5], rtol=1e-14)
+
+
 def test_incremental_sort_with_crowded_gap():
     """Atoms aimed at one cell: on the second step nearly every atom changes
     cell into the same cell, so one gap between two stays receives ~all the
@@ -279,6 +319,49 @@ def test_batched_trap_equals_standalone():
             total_coll += int(l1["collisions"].sum())
     assert total_coll > 0
     assert int(log["n_evap"].sum()) > 0
+
+
+def test_reads_after_a_removing_step():
+    """Host reads right after a step whose move removed atoms in several traps:
+    the removed atoms keep their slots (dead key) until the next sort, and
+    every read skips them.  dsmc_observe per trap equals the last log record
+    (same doubles, another summation order), its alive counts equal the trap
+    ranges of copy_out, and the survivors' velocities from copy_out give the
+    same sums; no survivor lies above the cut."""
+    traps = []
+    for n, dBdz, T in [(3000, 1.5, 30e-6), (20000, 4.0, 8e-6), (7000, 2.5, 15e-6), (12000, 3.0, 20e-6)]:
+        t = configs.quadrupole_trap(n, configs.cells_for_trap(n), T=T, dBdz=dBdz, n_real=3e6)
+        # a cut ramped down through the thermal tail: atoms go in every step
+        t.eta_start, t.eta_end, t.T_cut, t.eta_steps = 5.0, 2.0, T, 40
+        traps.append(t)
+    run = configs.Run(traps=traps, precision="fp32", energy_cut=True, dt=1e-6, seed=13,
+                      name="reads-after-removal")
+    for steps in (25, 26):   # last step replayed from the graph, then eager
+        print(f"reads after a removing step, {steps} steps", flush=True)
+        with Simulation(run) as sim:
+            sim.init_distribution()
+            sim.step(steps)
+            last = sim.obs_log()[-1]
+            obs = sim.observe()
+            ts = sim.copy_out(_abi.FIELD_TRAP_START)
+            vel = sim.copy_out(_abi.FIELD_VEL)
+            pos = sim.copy_out(_abi.FIELD_POS)
+            ids = sim.copy_out(_abi.FIELD_ATOM_ID)
+            info = sim.info()
+        assert int((last["n_evap"] > 0).sum()) >= 2, last["n_evap"]
+        assert info.n_alive == ts[-1] == len(ids) == len(np.unique(ids))
+        for t, tr in enumerate(run.traps):
+            o = obs[t]
+            assert o.n_alive == last["n_alive"][t] == ts[t + 1] - ts[t]
+            np.testing.assert_allclose([o.v2[0], o.v2[1], o.v2[2], o.ep], list(last["v2"][t]) + [last["ep"][t]],
+                                       rtol=1e-12)
+            v = vel[ts[t]:ts[t + 1]]
+            np.testing.assert_allclose(list(o.v2), (v ** 2).sum(axis=0), rtol=1e-12)
+            x, y, z = pos[ts[t]:ts[t + 1]].T
+            E = 0.5 * run.mass * (v ** 2).sum(axis=1) + tr.mu * tr.dBdz * np.sqrt(x * x + y * y + 4 * z * z)
+            f = min((steps - 1) / tr.eta_steps, 1.0)
+            cut = (tr.eta_start + (tr.eta_end - tr.eta_start) * f) * configs.KB * tr.T_cut
+            assert E.max() <= cut * (1 + 1e-6)
 
 
 def test_energy_cut_survivors(orc):
