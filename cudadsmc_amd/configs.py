@@ -22,7 +22,9 @@ spin_evaporation  N = 4e6 at 50 uK with spin vectors along B, 20 spin substeps o
                   over 2e5 steps.
 batched_traps     256 independent quadrupole traps, N log-uniform in [1e3, 1e5],
                   B' ~ U[1, 5] T/m, T ~ U[5, 50] uK, 32^3..64^3 cells chosen per
-                  trap, per-trap energy cut.
+                  trap, and per trap an energy-cut schedule: eta ramps linearly
+                  from eta_start ~ U[7, 10] down by U[1, 2] over U[1e4, 5e4]
+                  steps (E_cut = eta kB T).
 """
 from __future__ import annotations
 
@@ -204,14 +206,19 @@ def batched_traps(n_traps: int = 256, seed: int = 1, steps: int = 5000) -> Run:
     import numpy as np
 
     rng = np.random.default_rng(seed)
+    rng_cut = np.random.default_rng([seed, 1])
     traps = []
     for _ in range(n_traps):
         n = int(math.floor(math.exp(rng.uniform(math.log(1e3), math.log(1e5)))))
         dBdz = float(rng.uniform(1.0, 5.0))
         T = float(rng.uniform(5e-6, 50e-6))
-        eta = float(rng.uniform(6.0, 10.0))
+        eta0 = float(rng.uniform(7.0, 10.0))
+        # the ramp from a second stream, so that N, B' and T stay those of
+        # the seed's first stream
+        eta1 = eta0 - float(rng_cut.uniform(1.0, 2.0))
+        ramp = int(rng_cut.uniform(1e4, 5e4))
         tr = quadrupole_trap(n, cells_for_trap(n), T=T, dBdz=dBdz, n_real=float(n))
-        tr.eta_start, tr.eta_end, tr.T_cut, tr.eta_steps = eta, eta, T, 0
+        tr.eta_start, tr.eta_end, tr.T_cut, tr.eta_steps = eta0, eta1, T, ramp
         traps.append(tr)
     return Run(traps=traps, precision="fp32", energy_cut=True, dt=1e-6, steps=steps, seed=seed,
                name=f"batched-{n_traps}-traps")

@@ -8,7 +8,8 @@
 //   cudadsmc --quadrupole N Nc STEPS [fp32|fp64]
 //                                       spherical quadrupole, B' = 2.5 T/m, 20 uK,
 //                                       grid +-5 thermal radii, dt = 1e-6 s
-//   options: --out FILE, --prints K
+//   options: --out FILE, --prints K, --nreal X (real atoms of the quadrupole
+//            run, default 3e6 as h_N of initialSystemParameters.cuh:26)
 //
 // Output: with libdsmc_h5 (built where libhdf5 is installed) the HDF5 file of
 // main.cu:202-577, default outputData.h5 (main.cu:33-34), one frame of the
@@ -95,11 +96,13 @@ int main(int argc, const char *argv[])
 
     std::vector<std::string> pos_args;
     bool quad = false;
+    double n_real = 3e6;   // h_N (initialSystemParameters.cuh:26)
     std::string precision = "fp64";
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--out" && i + 1 < argc) out = argv[++i];
         else if (a == "--prints" && i + 1 < argc) prints = std::atoi(argv[++i]);
+        else if (a == "--nreal" && i + 1 < argc) n_real = std::atof(argv[++i]);
         else if (a == "--quadrupole") quad = true;
         else pos_args.push_back(a);
     }
@@ -156,7 +159,7 @@ int main(int argc, const char *argv[])
         trap.dBdz = dBdz;
         trap.mu = mu;
         trap.T = T;
-        trap.n_real = 3e6;
+        trap.n_real = n_real;
         cfg.precision = precision == "fp32" ? DSMC_FP32_KAHAN : DSMC_FP64_PLAIN;
         cfg.pair_mode = DSMC_PAIR_UNIFORM;
         cfg.lpc = 1;

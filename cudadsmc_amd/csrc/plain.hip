@@ -86,7 +86,10 @@ __global__ __launch_bounds__(kBlock) void k_collide_plain(
     for (int cell = blockIdx.x * kBlock + threadIdx.x; cell < ncells;
          cell += gridDim.x * kBlock) {
         const int n = prefix[cell + 1] - prefix[cell];
-        if (n <= 1) continue;
+        // every cell's count is that of this call: collisions.cu:308,335
+        // reset it only in cells of two or more atoms (a cell that empties
+        // keeps a stale count there)
+        if (n <= 1) { coll_count[cell] = 0; continue; }
         const double Mc = 0.5 * (n - 1) * n;
         const double X = Mc * t.alpha * lpc * cp.dt * sigvrmax[cell] / cellVolume;
         const double lambda = ceil(X) / Mc;

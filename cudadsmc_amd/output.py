@@ -7,8 +7,9 @@ CUDADSMC/src/hdf5Helpers.cu:13-163: group "/atomData", rank-3 datasets
 of main.cu:202-383 written once per print (main.cu:526-576).
 
 `energy_check` and `temperature` restate CUDADSMC/energyChecker.py:118-124,
-170-185 (Python 3, reading the file through the same library instead of
-h5py).
+170-185, `collision_rate` CUDADSMC/collisionChecker.py:56-64 and
+`collision_rate_cumulative` CUDADSMC/collisionComparison.py:70-75 (Python 3,
+reading the file through the same library instead of h5py).
 """
 from __future__ import annotations
 
@@ -176,3 +177,29 @@ def temperature(frames: Dict[str, np.ndarray], mass: float = MRB) -> np.ndarray:
         v = frames["velocities"][:n, :, i]
         out[i] = mass * np.mean(np.sum(v * v, axis=1)) / (3.0 * KB)
     return out
+
+
+def collision_rate(frames: Dict[str, np.ndarray]) -> np.ndarray:
+    """collisionChecker.py:56-58: per frame, the collisions counted in the
+    cells (collisionCount holds the counts of the frame's last collision
+    step, collisions.cu:335-436) over the frame interval and the atom number:
+    events per atom per second when frames are one collision step apart."""
+    time = frames["simuatedTime"][0, 0, :]
+    total = frames["collisionCount"][:, 0, :].sum(axis=0).astype(np.float64)
+    return total / (time[1] - time[0]) / frames["atomNumber"][0, 0, :]
+
+
+def collision_rate_cumulative(frames: Dict[str, np.ndarray]) -> np.ndarray:
+    """collisionComparison.py:70-72: the rate from counts that accumulate over
+    the run, differentiated in time (np.gradient) and divided by the atom
+    number."""
+    time = frames["simuatedTime"][0, 0, :]
+    total = frames["collisionCount"][:, 0, :].sum(axis=0).astype(np.float64)
+    return np.gradient(total, time[1] - time[0]) / frames["atomNumber"][0, 0, :]
+
+
+def rate_summary(rate: np.ndarray, skip: int = 10):
+    """Mean and standard deviation of a rate over frames skip .. -2
+    (collisionChecker.py:60-61 and collisionComparison.py:74-75 use 10)."""
+    r = np.asarray(rate)[skip:-1]
+    return float(np.mean(r)), float(np.std(r))

@@ -298,7 +298,10 @@ long long orc_collide(const orc_params *p, orc_d3 *vel, double *sigvrmax,
 #pragma omp parallel for schedule(dynamic, 512) reduction(+ : total)
     for (int cell = 0; cell < ncells; ++cell) {
         int n = prefix[cell + 1] - prefix[cell];
-        if (n <= 1) continue;
+        /* collisions.cu:308,335 reset the count only in cells of two or more
+         * atoms, so a cell that empties keeps a stale count; the count of
+         * every cell here is that of this call */
+        if (n <= 1) { collisionCount[cell] = 0; continue; }
         double Mc = 0.5 * (n - 1) * n;
         double X = Mc * p->alpha * lpc * p->dt * sigvrmax[cell] / cellVolume;
         double lambda = ceil(X) / Mc;

@@ -84,3 +84,34 @@ def test_driver_writes_the_reference_file(tmp_path):
     assert fr["atomCount"].shape == (8 ** 3 + 1, 1, 3)
     t = fr["simuatedTime"][0, 0, :]
     assert t[0] == 0.0 and np.all(np.diff(t) > 0)
+
+
+def test_driver_file_collision_rate_against_analytic(tmp_path):
+    """collisionChecker.py's rate read from the C++ driver's HDF5 file of a
+    quadrupole run (N = 1e4 test particles for 2e8 real atoms at 20 uK,
+    B' = 2.5 T/m, 35^3 cells, FP64 form, a frame after every step) against
+    the analytic trap rate (collisionChecker.py:34) within 4 sigma + 3 %
+    (the 35^3 grid lowers the rate by about 1 %).  The gas is dilute there
+    (about 5e3 events per atom per second: a mean free time of 200 steps and
+    a mean free path above the cell size), as in the reference's own checker
+    run (N = 1e6, 100^3 cells)."""
+    out = tmp_path / "quad.h5"
+    exe = ROOT / "cudadsmc_amd" / "bin" / "cudadsmc"
+    n, nc, steps, n_real, skip = 10000, 35, 120, 2e8, 20
+    cmd = [str(exe), "--quadrupole", str(n), str(nc), str(steps), "fp64", "--prints", str(steps),
+           "--nreal", str(n_real), "--out", str(out)]
+    print("driver: " + " ".join(cmd[1:]), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    fr = output.read_frames(str(out))
+    assert fr["collisionCount"].shape == (nc ** 3 + 1, 1, steps + 1)
+    rate = output.collision_rate(fr)
+    mean, std = output.rate_summary(rate, skip=skip)
+    events = int(fr["collisionCount"][:, 0, skip:-1].sum())
+    expect = configs.analytic_trap_rate(n_real, 20e-6, 2.5)
+    half = len(rate[skip:-1]) // 2
+    print(f"rate {mean:.1f} +- {std:.1f} /s vs analytic {expect:.1f} /s ({events} events); "
+          f"halves {np.mean(rate[skip:skip + half]):.1f}, {np.mean(rate[skip + half:-1]):.1f}",
+          flush=True)
+    assert events > 3000
+    assert abs(mean / expect - 1.0) < 4.0 / np.sqrt(events) + 0.03

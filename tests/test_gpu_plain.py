@@ -153,6 +153,57 @@ def test_init_distribution_matches_oracle(orc):
     np.testing.assert_allclose(vel, st["vel"], rtol=1e-13, atol=1e-18)
 
 
+def _bloch(up, dn):
+    return np.stack([2 * (up[:, 0] * dn[:, 0] + up[:, 1] * dn[:, 1]),
+                     2 * (up[:, 0] * dn[:, 1] - up[:, 1] * dn[:, 0]),
+                     (up ** 2).sum(1) - (dn ** 2).sum(1)], axis=1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("kind", ["ip", "box", "quadrupole"])
+def test_init_samplers_match_oracle(orc, kind, precision):
+    """The device samplers of setUp.cu:129-252 against the oracle's, from the
+    same Philox draws: the Ioffe-Pritchard rejection sampler
+    (setUp.cu:178-205) with spins aligned to the local field
+    (setUp.cu:236-252), the periodic box (uniform positions, per-axis
+    Maxwell velocities) and the quadrupole; in both numerical forms (the
+    FP32 form stores the FP64 sample as value + remainder, and its spin as
+    the Bloch vector of the aligned spinor).  To 1e-13 (the device log, exp
+    and sin differ from glibc in the last ulp)."""
+    n = 20000
+    if kind == "ip":
+        run = configs.reference(Nc=40, N=n).with_(mcwf=False, spin_loss=False)
+        if precision == "fp32":
+            run = run.with_(lpc=1)
+    elif kind == "box":
+        L = 1e-3
+        trap = configs.Trap(kind="box", n_atoms=n, nc=(32, 32, 32), half_width=(L / 2,) * 3,
+                            T=(40e-6 + 2 * 20e-6) / 3, T_axis=(40e-6, 20e-6, 20e-6), n_real=1e10,
+                            mu=0.0)
+        run = configs.Run(traps=[trap], dt=1e-6, seed=11)
+    else:
+        run = quad_run(n=n).with_(spin=True)
+    run = run.with_(precision=precision, obs_capacity=16)
+    t = run.traps[0]
+    print(f"init sampler {kind} {precision}", flush=True)
+    with Simulation(run) as sim:
+        sim.init_distribution()
+        pos, ids = by_label(sim, _abi.FIELD_POS)
+        vel, _ = by_label(sim, _abi.FIELD_VEL)
+        spin = by_label(sim, _abi.FIELD_SPIN)[0] if run.spin else None
+    assert np.array_equal(np.sort(ids), np.arange(n))
+    p = orc.params_from(run.with_(precision="fp64"))
+    st = orc.init_distribution(p, n, t.T, None if not any(t.T_axis) else t.T_axis, spin=run.spin)
+    np.testing.assert_allclose(pos, st["pos"], rtol=1e-13, atol=1e-20)
+    np.testing.assert_allclose(vel, st["vel"], rtol=1e-13, atol=1e-18)
+    if run.spin:
+        if precision == "fp64":
+            np.testing.assert_allclose(spin[:, :2], st["psi_up"], rtol=0, atol=1e-13)
+            np.testing.assert_allclose(spin[:, 2:], st["psi_dn"], rtol=0, atol=1e-13)
+        else:
+            np.testing.assert_allclose(spin, _bloch(st["psi_up"], st["psi_dn"]), rtol=0, atol=1e-6)
+
+
 def test_golden_index_fixture(orc):
     """tests/golden/index_fixture.npz: 4096 positions of the 35^3 quadrupole
     grid with atoms on cell faces, one ulp either side, on the grid edges,

@@ -81,3 +81,44 @@ def test_energy_checker_on_the_trap_axis():
     assert Ek[1] == pytest.approx(ek, rel=1e-12) and Ep[1] == pytest.approx(ep, rel=1e-12)
     dE, ok = output.energy_check(frames, B0=B0, dBdx=Bp, d2Bdx2=Bpp)
     assert dE == pytest.approx(abs(ek) / ep * 100.0, rel=1e-9) and not ok
+
+
+def _rate_file(path, per_frame, time, n_atoms, cells=64):
+    """A file of the reference's layout holding only the datasets the
+    collision checkers read."""
+    T = len(time)
+    with output.H5Writer(str(path)) as w:
+        w.dataset("/atomData/collisionCount", cells + 1, 1, output.INT)
+        w.dataset("/atomData/simuatedTime", 1, 1, output.DOUBLE)
+        w.dataset("/atomData/atomNumber", 1, 1, output.INT)
+        for i in range(T):
+            w.append("/atomData/collisionCount", per_frame[:, i:i + 1], output.INT)
+            w.append("/atomData/simuatedTime", np.array([[time[i]]]), output.DOUBLE)
+            w.append("/atomData/atomNumber", np.array([[n_atoms[i]]], np.int32), output.INT)
+    return {k: output.read(str(path), f"/atomData/{k}", t)
+            for k, t in (("collisionCount", output.INT), ("simuatedTime", output.DOUBLE),
+                         ("atomNumber", output.INT))}
+
+
+def test_collision_rate_readers(tmp_path):
+    """The two collision-rate checkers on files of a Poisson process of known
+    rate: collisionChecker.py reads per-frame counts, collisionComparison.py
+    cumulative ones; both must return the rate (events per atom per second)
+    within the Poisson error, and the per-frame reader on cumulative counts
+    (or the other way round) must not."""
+    rng = np.random.default_rng(9)
+    R, N, dt, T, cells = 250.0, 20000, 1e-4, 400, 64
+    time = np.arange(T) * dt
+    n = np.full(T, N)
+    lam = R * N * dt                      # events per frame
+    counts = rng.poisson(lam / cells, size=(cells + 1, T)).astype(np.int32)
+    counts[cells] = 0                     # the overflow cell holds none
+    per = _rate_file(tmp_path / "per.h5", counts, time, n, cells)
+    cum = _rate_file(tmp_path / "cum.h5", np.cumsum(counts, axis=1).astype(np.int32), time, n, cells)
+    m1, s1 = output.rate_summary(output.collision_rate(per))
+    m2, s2 = output.rate_summary(output.collision_rate_cumulative(cum))
+    err = R / np.sqrt(lam * (T - 11))
+    assert abs(m1 - R) < 5 * err and abs(m2 - R) < 5 * err
+    assert s1 == pytest.approx(R / np.sqrt(lam), rel=0.15)
+    assert output.rate_summary(output.collision_rate(cum))[0] > 10 * R
+    assert output.rate_summary(output.collision_rate_cumulative(per))[0] < 0.1 * R
