@@ -198,9 +198,10 @@ struct dsmc_ctx {
 
     ~dsmc_ctx()
     {
+        // drain the stream before destroying a graph it may still be running
+        if (stream) (void)hipStreamSynchronize(stream);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
-        if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : allocs) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -235,7 +236,9 @@ static TrapDev make_trapdev(const dsmc_config &c, const dsmc_trap &t, int index,
         d.len[k] = (2.0 * t.half_width[k]) / (double)t.nc[k];
         d.inv_len[k] = 1.0 / d.len[k];
         d.f_hw[k] = (float)t.half_width[k];
+        d.f_inv_len[k] = (float)d.inv_len[k];
     }
+    d.f32_index = t.nc[0] <= 256 && t.nc[1] <= 256 && t.nc[2] <= 256;
     d.n_cells = t.nc[0] * t.nc[1] * t.nc[2];
     d.key_base = key_base;
     d.atom_begin = atom_begin;
@@ -261,6 +264,7 @@ static TrapDev make_trapdev(const dsmc_config &c, const dsmc_trap &t, int index,
     d.f_B0 = (float)t.B0; d.f_dBdx = (float)t.dBdx; d.f_d2Bdx2 = (float)t.d2Bdx2;
     d.f_dBdz = (float)t.dBdz;
     d.f_neg_mu_over_m = (float)d.neg_mu_over_m;
+    d.f_mu_dBdz = (float)(t.mu * t.dBdz);
     return d;
 }
 
@@ -610,6 +614,7 @@ static MoveParams move_params(const dsmc_ctx *x)
     mp.dt = x->cfg.dt;
     mp.f_dt = (float)x->cfg.dt;
     mp.mass = x->cfg.mass;
+    mp.f_half_mass = (float)(0.5 * x->cfg.mass);
     mp.gs = x->cfg.gs;
     mp.spin = x->cfg.spin;
     mp.substeps = std::max(1, x->cfg.spin_substeps);
@@ -764,15 +769,25 @@ static void fused_step(dsmc_ctx *x)
 
 static bool graph_ok(const dsmc_ctx *x) { return x->fp32 && x->cfg.use_graph && !x->prof.enabled; }
 
+// Destroy the captured graph.  Its earlier launches may still be in flight on
+// the stream, so the stream is drained first: an executable graph is not
+// destroyed while the device may still run it.
+static void drop_graph(dsmc_ctx *x)
+{
+    if (!x->graph_exec && !x->graph) return;
+    DSMC_HIP(hipStreamSynchronize(x->stream));
+    if (x->graph_exec) (void)hipGraphExecDestroy(x->graph_exec);
+    if (x->graph) (void)hipGraphDestroy(x->graph);
+    x->graph_exec = nullptr;
+    x->graph = nullptr;
+}
+
 static void fused_graph_steps(dsmc_ctx *x, int pairs)
 {
     if (x->graph_exec && (x->graph_srt != x->srt_cur || x->graph_ssel != x->ssel ||
                           x->graph_tsel != x->tsel)) {
         // captured with the other sort (step, trap-range) buffer current
-        (void)hipGraphExecDestroy(x->graph_exec);
-        (void)hipGraphDestroy(x->graph);
-        x->graph_exec = nullptr;
-        x->graph = nullptr;
+        drop_graph(x);
     }
     if (!x->graph_exec) {
         const int cur0 = x->cur;
@@ -1689,12 +1704,7 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
     DSMC_HIP(hipStreamSynchronize(ctx->stream));
     if (mp.count_moved) ctx->srt_valid = false;
     if (ctx->block != best_bs || ctx->apt != best_apt) {
-        if (ctx->graph_exec) {
-            (void)hipGraphExecDestroy(ctx->graph_exec);
-            (void)hipGraphDestroy(ctx->graph);
-            ctx->graph_exec = nullptr;
-            ctx->graph = nullptr;
-        }
+        drop_graph(ctx);
         // the per-block counts of the last move are for the old block size
         ctx->srt_valid = false;
     }

@@ -55,9 +55,11 @@ struct TrapDev {
     uint32_t stream;       // Philox stream of this trap
     // FP32 copies for the precision form
     float  f_hw[3];
+    float  f_inv_len[3];   // 1 / len, for the FP32 index path
     float  f_B0, f_dBdx, f_d2Bdx2, f_dBdz;
     float  f_neg_mu_over_m;
-    float  f_pad;
+    float  f_mu_dBdz;      // quadrupole: U = mu dBdz rho
+    int    f32_index;      // every nc <= 256: the FP32 index path is exact (cell_of_f32)
 };
 
 // ---------------------------------------------------------------------------
@@ -246,6 +248,32 @@ __device__ __forceinline__ int cell_of_fast(const TrapDev &t, double x, double y
         if (frac < 1e-9 || frac > 1.0 - 1e-9) q = floor(num / t.len[d]);
         if (!(q > -1.0 && q < (double)t.nc[d])) return t.n_cells;
         idx[d] = (int)q;
+    }
+    return idx[2] * t.nc[0] * t.nc[1] + idx[1] * t.nc[0] + idx[0];
+}
+
+// Same result as cell_of for the position p + rp (value + Kahan remainder),
+// mostly in FP32: q' = (p + hw) / len from the FP32 value differs from the
+// exact FP64 quotient of p + rp by less than 4 2^-24 nc (below 6.2e-5 of a
+// cell for nc <= 256, which t.f32_index guarantees); only when q' lies
+// within 1e-4 of an integer is the reference formula evaluated in FP64 on
+// p + rp.  NaN fails the range test in both branches.
+__device__ __forceinline__ int cell_of_f32(const TrapDev &t, const float (&p)[3], const float (&rp)[3])
+{
+    int idx[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float qf = (p[d] + t.f_hw[d]) * t.f_inv_len[d];
+        const float fl = floorf(qf);
+        const float fr = qf - fl;
+        if (fr < 1e-4f || fr > 1.0f - 1e-4f) {
+            const double q = floor(((double)p[d] + (double)rp[d] - t.gmin[d]) / t.len[d]);
+            if (!(q > -1.0 && q < (double)t.nc[d])) return t.n_cells;
+            idx[d] = (int)q;
+        } else {
+            if (!(fl > -1.0f && fl < (float)t.nc[d])) return t.n_cells;
+            idx[d] = (int)fl;
+        }
     }
     return idx[2] * t.nc[0] * t.nc[1] + idx[1] * t.nc[0] + idx[0];
 }

@@ -801,12 +801,22 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
     kahan_add(m.f.vy, m.f.rvy, a1.y * hdt);
     kahan_add(m.f.vz, m.f.rvz, a1.z * hdt);
 
-    const double X = (double)m.f.px + (double)m.f.rpx, Y = (double)m.f.py + (double)m.f.rpy,
-                 Z = (double)m.f.pz + (double)m.f.rpz;
-    const double VX = (double)m.f.vx + (double)m.f.rvx, VY = (double)m.f.vy + (double)m.f.rvy,
-                 VZ = (double)m.f.vz + (double)m.f.rvz;
-    const double ek = 0.5 * mp.mass * (VX * VX + VY * VY + VZ * VZ);
-    const double ep = potential_energy(tr, X, Y, Z);
+    // Per-atom energies, cut test and cell key in FP32 (the FP64 sums and the
+    // exact index near a cell face aside): the squared speeds and the
+    // quadrupole potential from the FP32 values; the Ioffe-Pritchard
+    // potential mu (|B| - B0) in FP64, where |B| - B0 cancels.
+    const float v2x = m.f.vx * m.f.vx, v2y = m.f.vy * m.f.vy, v2z = m.f.vz * m.f.vz;
+    const float ekf = mp.f_half_mass * (v2x + v2y + v2z);
+    double ep;
+    if (tr.kind == kTrapQuad) {
+        const float r2 = m.f.px * m.f.px + m.f.py * m.f.py + 4.0f * (m.f.pz * m.f.pz);
+        ep = (double)(tr.f_mu_dBdz * sqrtf(r2));
+    } else if (tr.kind == kTrapIP) {
+        ep = potential_energy(tr, (double)m.f.px + (double)m.f.rpx, (double)m.f.py + (double)m.f.rpy,
+                              (double)m.f.pz + (double)m.f.rpz);
+    } else {
+        ep = 0.0;
+    }
     lost = false;
     if (mp.spin_loss) {
         F3 Bn;
@@ -816,21 +826,34 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
     if (mp.energy_cut && tr.kT_cut > 0.0) {
         const double f = tr.eta_steps > 0 ? fmin((double)step / (double)tr.eta_steps, 1.0) : 1.0;
         const double eta = tr.eta_start + (tr.eta_end - tr.eta_start) * f;
-        lost = lost || (ek + ep > eta * tr.kT_cut);
+        lost = lost || ((double)ekf + ep > eta * tr.kT_cut);
     }
     uint32_t kout;
     moved = false;
     if (lost) {
+        // a removed atom's record in FP64 from value + remainder (rare)
+        const double X = (double)m.f.px + (double)m.f.rpx, Y = (double)m.f.py + (double)m.f.rpy,
+                     Z = (double)m.f.pz + (double)m.f.rpz;
+        const double VX = (double)m.f.vx + (double)m.f.rvx, VY = (double)m.f.vy + (double)m.f.rvy,
+                     VZ = (double)m.f.vz + (double)m.f.rvz;
         kout = mp.dead_key;
         const size_t id = (size_t)m.id;
         evap_pos[3 * id + 0] = (float)X; evap_pos[3 * id + 1] = (float)Y;
         evap_pos[3 * id + 2] = (float)Z;
         evap_vel[3 * id + 0] = (float)VX; evap_vel[3 * id + 1] = (float)VY;
         evap_vel[3 * id + 2] = (float)VZ;
-        lost_e = ek + ep;
+        lost_e = 0.5 * mp.mass * (VX * VX + VY * VY + VZ * VZ) + potential_energy(tr, X, Y, Z);
     } else {
-        kout = (uint32_t)(tr.key_base + cell_of_fast(tr, X, Y, Z));
-        vals[0] = VX * VX; vals[1] = VY * VY; vals[2] = VZ * VZ;
+        int c;
+        if (tr.f32_index) {
+            const float p[3] = { m.f.px, m.f.py, m.f.pz }, rp[3] = { m.f.rpx, m.f.rpy, m.f.rpz };
+            c = cell_of_f32(tr, p, rp);
+        } else {
+            c = cell_of_fast(tr, (double)m.f.px + (double)m.f.rpx, (double)m.f.py + (double)m.f.rpy,
+                             (double)m.f.pz + (double)m.f.rpz);
+        }
+        kout = (uint32_t)(tr.key_base + c);
+        vals[0] = (double)v2x; vals[1] = (double)v2y; vals[2] = (double)v2z;
         vals[3] = ep;
         if (mp.count_moved) moved = kout != m.kold;
     }
@@ -1316,8 +1339,10 @@ void k_move_fused(
 // the trap's observables to the log and clears the per-step accumulators.
 // The next sort puts the atoms removed now at the end, so trap t then starts
 // at its first slot minus the atoms removed before it, and the alive count is
-// the last trap's end minus all removed atoms.  With P = 1 (up to 2048 move
-// blocks per trap) no block waits for or reads what another writes.  Block
+// the last trap's end minus all removed atoms.  P is about one block per 256
+// move blocks of the largest trap; with P > 1 the last of a trap's blocks to
+// finish (an atomic count) combines the others' partial sums, so no block
+// waits for another.  Block
 // (0, 0) writes the next step number to the other counter.  Blocks from P T
 // on run the next incremental re-sort's first phase (resort_prep_block).
 __global__ __launch_bounds__(256) void k_obs_finalize(
@@ -1824,8 +1849,10 @@ void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, double
                          int *n_next, const ResortPrepArgs &prep, hipStream_t st,
                          Profiler *prof)
 {
+    // blocks per trap: about 256 move-block records each (a single block
+    // summing thousands of records is a chain of dependent load rounds)
     const int blocks = div_up(max_trap_atoms > 0 ? max_trap_atoms : 1, move_block) + 1;
-    const int P = std::min(kMaxFinalizeP, div_up(blocks, 2048));
+    const int P = std::min(kMaxFinalizeP, div_up(blocks, 256));
     DSMC_LAUNCH(prof, "obs_finalize", (double)n_traps * 64.0, k_obs_finalize,
                 P * n_traps + prep.blocks, 256, st, n_traps, P, move_block, 0.5 * mass,
                 trap_start, trap_next, blk_dead, mask_dead, (const int *)rec.trap,

@@ -96,6 +96,7 @@ struct MoveParams {
     double mass;
     double gs;
     float  f_dt;
+    float  f_half_mass;
     int    spin;
     int    substeps;
     int    interp;
