@@ -202,7 +202,10 @@ def pmc_child(steps):
         sim.init_distribution()
         sim.step(steps)
         sim.sync()
-    print(json.dumps({"copy_bytes": cal["bytes"], "gather": gat}), flush=True)
+        lg = sim.obs_log()
+    # atoms each step's move read: alive after it plus removed by it
+    moved = (lg["n_alive"].sum(axis=1) + lg["n_evap"].sum(axis=1)).astype(int).tolist()
+    print(json.dumps({"copy_bytes": cal["bytes"], "gather": gat, "atoms_moved": moved}), flush=True)
 
 
 def _pmc_values(db_glob):

@@ -435,7 +435,7 @@ constexpr int kMergePer = kStayTile / 256;   // slots per thread
 
 __global__ __launch_bounds__(256) void k_ins_merge(
     const uint32_t *__restrict__ key_new, const uint64_t *__restrict__ mask_m,
-    const uint64_t *__restrict__ mask_d, const int *__restrict__ n_dev,
+    const uint64_t *__restrict__ mask_d, const int *__restrict__ n_dev, int n_cap,
     const uint32_t *__restrict__ pre_m, const uint32_t *__restrict__ pre_d,
     const uint32_t *__restrict__ fb_k, const uint32_t *__restrict__ fb_v,
     const uint32_t *__restrict__ fb_q, const uint32_t *__restrict__ ov_k,
@@ -445,52 +445,69 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     uint32_t *__restrict__ srt_q, uint32_t *__restrict__ alt_k, uint32_t *__restrict__ alt_v,
     uint32_t *__restrict__ alt_q, uint32_t *__restrict__ out_k, uint32_t *__restrict__ out_v)
 {
-    __shared__ uint32_t lk[kBucketLds], lv[kBucketLds], lq[kBucketLds];
+    // the bucket's movers: (key << 32 | slot), the order they are ranked in,
+    // and insertion points
+    __shared__ __align__(16) uint64_t lkv[kBucketLds];
+    __shared__ __align__(16) uint32_t lq[kBucketLds];
     __shared__ uint32_t H[kStayTile];      // histogram h, then the scan E
     __shared__ uint64_t stw[kTileWords];   // stay bits of the tile's words
     __shared__ uint32_t wst[kTileWords];   // stays of the tile before each word
     __shared__ uint32_t tmp[256 / kWave + 1];
     __shared__ int n_big;   // movers gathered for a big bucket
-    const int n = *n_dev;
     const int t = blockIdx.x;
-    if (t > (n >> kStayShift)) return;   // whole block leaves together
     const int tid = threadIdx.x;
     const int g0 = t * kStayTile;
-    const int ns = min(kStayTile, n - g0);   // slots of the tile (0 for the bucket at n)
     const int p0 = tid * kMergePer;   // positions p0 .. p0 + 7 in the scan
-    // keys of positions tid + 256 u (coalesced), loaded first so that the
-    // loads overlap the bucket's loading
+    // Every load that does not need the alive count is issued together with
+    // it, in one round: the tile's keys at positions tid + 256 u (coalesced;
+    // up to the capacity, those past the alive slots are never used), its
+    // mask words, counts and bucket size, the bucket counts of the earlier
+    // tiles, and this thread's first mover of the bucket's room (used when
+    // the bucket holds it).
     uint32_t kk[kMergePer];
 #pragma unroll
     for (int u = 0; u < kMergePer; ++u) {
         const int p = tid + 256 * u;
-        kk[u] = p < ns ? key_new[g0 + p] : 0u;
+        kk[u] = g0 + p < n_cap ? key_new[g0 + p] : 0u;
     }
-    // the tile's counts, and the bucket counts of earlier tiles, in flight
-    // with the keys
+    const int n = *n_dev;
     const uint32_t pm = pre_m[t], pd = pre_d[t];
     const int b = (int)bcnt[t];
+    const int n_dead = counts[2];
+    uint64_t wm = 0ull, wd = 0ull;   // mask words of the tile (in bounds up to the last tile + 1)
+    if (tid < kTileWords) {
+        wm = mask_m[(g0 >> 6) + tid];
+        wd = mask_d[(g0 >> 6) + tid];
+    }
+    const size_t fb0 = (size_t)t * kBucketCap;   // the bucket's fixed room
+    const uint32_t mk = fb_k[fb0 + tid], mv = fb_v[fb0 + tid], mq = fb_q[fb0 + tid];
     uint32_t bsum = 0;
     {
         const uint4 *c4 = (const uint4 *)bcnt;
         const int nv = t >> 2;
-        for (int v = tid; v < nv; v += 256) {
-            const uint4 c = c4[v];
-            bsum += c.x + c.y + c.z + c.w;
+        for (int v0 = tid; v0 < nv; v0 += 4 * 256) {
+            uint4 c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int v = v0 + u * 256;
+                c[u] = v < nv ? c4[v] : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) bsum += c[u].x + c[u].y + c[u].z + c[u].w;
         }
         if (tid < (t & 3)) bsum += bcnt[4 * nv + tid];
     }
+    if (t > (n >> kStayShift)) return;   // whole block leaves together
+    const int ns = min(kStayTile, n - g0);   // slots of the tile (0 for the bucket at n)
     if (tid < kTileWords) {
         const int l0 = tid * 64;
         uint64_t st = 0ull, dd = 0ull;
         uint32_t od = 0u;
         if (l0 < ns) {
-            const int w = (g0 >> 6) + tid;
             const uint64_t in = ns - l0 >= 64 ? ~0ull : (1ull << (ns - l0)) - 1ull;
-            const uint64_t md = mask_d[w];
-            st = ~(mask_m[w] | md) & in;
-            dd = md & in;
-            od = (uint32_t)(n - counts[2]) + pd;
+            st = ~(wm | wd) & in;
+            dd = wd & in;
+            od = (uint32_t)(n - n_dead) + pd;
         }
         stw[tid] = st;
         wst[tid] = halfwave_exclusive_scan((uint32_t)__popcll(st));
@@ -503,11 +520,7 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     }
 #pragma unroll
     for (int u = 0; u < kMergePer; ++u) H[tid + 256 * u] = 0u;
-    const size_t fb0 = (size_t)t * kBucketCap;   // the bucket's fixed room
     const bool small = b <= kBucketCap;
-    // this thread's first mover of a small bucket, loaded before the scan
-    uint32_t mk = 0u, mv = 0u, mq = 0u;
-    if (small && tid < b) { mk = fb_k[fb0 + tid]; mv = fb_v[fb0 + tid]; mq = fb_q[fb0 + tid]; }
     // movers in earlier buckets
     uint32_t before;
     (void)block_exclusive_scan<256>(bsum, tmp, &before);   // syncs: H cleared, stay words in place
@@ -516,12 +529,14 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     const uint32_t *gk = nullptr, *gv = nullptr, *gq = nullptr;   // sorted big bucket
     if (small) {
         if (tid < b) {
-            lk[tid] = mk; lv[tid] = mv; lq[tid] = mq;
+            lkv[tid] = (uint64_t)mk << 32 | mv;
+            lq[tid] = mq;
             atomicAdd(&H[mq - (uint32_t)g0], 1u);
         }
         for (int e = tid + 256; e < b; e += 256) {
             const uint32_t q = fb_q[fb0 + e];
-            lk[e] = fb_k[fb0 + e]; lv[e] = fb_v[fb0 + e]; lq[e] = q;
+            lkv[e] = (uint64_t)fb_k[fb0 + e] << 32 | fb_v[fb0 + e];
+            lq[e] = q;
             atomicAdd(&H[q - (uint32_t)g0], 1u);
         }
     } else {
@@ -543,6 +558,7 @@ __global__ __launch_bounds__(256) void k_ins_merge(
             }
         }
         __syncthreads();
+        uint32_t *lk = (uint32_t *)lkv, *lv = lk + kBucketLds;   // scratch of the sort
         const int which = big_bucket_sort(ak, av, aq, ck, cv, cq, b, lk, lv, lq);
         gk = which ? ck : ak; gv = which ? cv : av; gq = which ? cq : aq;
         for (int r = tid; r < b; r += 256) atomicAdd(&H[gq[r] - (uint32_t)g0], 1u);
@@ -579,13 +595,25 @@ __global__ __launch_bounds__(256) void k_ins_merge(
             const uint32_t stp = (uint32_t)(stw[p >> 6] >> (p & 63)) & 1u;
             const uint32_t hp = (p + 1 < (uint32_t)kStayTile ? H[p + 1] : total) - ep - stp;
             uint32_t r = 0;   // rank among the movers inserted at the same point
+            const uint64_t kx = lkv[x];
             if (hp > 1) {
-                const uint32_t kx = lk[x], vx = lv[x];
-                for (int f = 0; f < b; ++f) r += lq[f] == q && kv_less(lk[f], lv[f], kx, vx);
+                // four movers per step, without branches: (key, slot) order is
+                // the order of key << 32 | slot
+                const uint4 *lq4 = (const uint4 *)lq;
+                const ulonglong2 *k2 = (const ulonglong2 *)lkv;
+#pragma unroll 2
+                for (int f = 0; f < b; f += 4) {
+                    const uint4 qq = lq4[f >> 2];
+                    const ulonglong2 ka = k2[f >> 1], kb = k2[(f >> 1) + 1];
+                    r += (uint32_t)((qq.x == q) & (ka.x < kx));
+                    r += (uint32_t)((qq.y == q) & (ka.y < kx) & (f + 1 < b));
+                    r += (uint32_t)((qq.z == q) & (kb.x < kx) & (f + 2 < b));
+                    r += (uint32_t)((qq.w == q) & (kb.y < kx) & (f + 3 < b));
+                }
             }
             const uint32_t o = w0 + ep + r;
-            out_k[o] = lk[x];
-            out_v[o] = lv[x];
+            out_k[o] = (uint32_t)(kx >> 32);
+            out_v[o] = (uint32_t)kx;
         }
     } else {
         for (int r = tid; r < b; r += 256) {
@@ -674,7 +702,7 @@ void resort_incremental(const uint32_t *key_new, const uint32_t *key_old, const 
                 stream, key_new, key_old, key_samp, key_tsamp, r.mov_v, r.n_cap, n_dev, r.counts,
                 r.bk_k, r.bk_v, r.bk_q, r.mov_k, r.mov_r, r.mov_q, r.bcnt);
     DSMC_LAUNCH(prof, "resort_merge", n * 12.0, k_ins_merge, div_up(r.n_cap, kStayTile) + 1, 256,
-                stream, key_new, mask_m, mask_d, n_dev, r.tile_cnt, r.tile_dead, r.bk_k, r.bk_v,
+                stream, key_new, mask_m, mask_d, n_dev, r.n_cap, r.tile_cnt, r.tile_dead, r.bk_k, r.bk_v,
                 r.bk_q, r.mov_k, r.mov_r, r.mov_q, r.bcnt, r.counts, dead_key, r.srt_k, r.srt_v,
                 r.srt_q, r.alt_k, r.alt_v, r.spare, out_k, out_v);
 }
