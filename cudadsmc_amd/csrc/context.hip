@@ -421,7 +421,8 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
         x->mask_moved = x->alloc<uint64_t>(resort_mask_words(N));
         x->mask_dead = x->alloc<uint64_t>(resort_mask_words(N));
         x->key_samp = x->alloc<uint32_t>(resort_mask_words(N));
-        x->key_tsamp = x->alloc<uint32_t>(resort_mask_words(N) / (kResortTile / 64) + 1);
+        // (+ 4: the locate reads the samples in 16-byte pieces)
+        x->key_tsamp = x->alloc<uint32_t>(resort_mask_words(N) / (kResortTile / 64) + 4);
         x->srt_k[0] = x->key_b; x->srt_v[0] = x->val_b;
         x->srt_k[1] = x->alloc<uint32_t>(N); x->srt_v[1] = x->alloc<uint32_t>(N);
         x->cell_start = x->alloc<uint32_t>(x->n_keys, false);

@@ -273,13 +273,29 @@ __global__ __launch_bounds__(256) void k_ins_locate(
     uint32_t *__restrict__ fb_q, uint32_t *__restrict__ ov_k, uint32_t *__restrict__ ov_v,
     uint32_t *__restrict__ ov_q, uint32_t *__restrict__ bcnt)
 {
-    __shared__ uint32_t lt[kMaxBuckets];
+    __shared__ __align__(16) uint32_t lt[kMaxBuckets];
     // the first round's mover slot and the tile samples are loaded with the
-    // counts (entries past them are never used)
+    // counts (entries past them are never used); the samples four per
+    // 16-byte load, up to four loads per thread in flight
     const int jf = blockIdx.x * 256 + threadIdx.x;
     uint32_t vn = jf < n_cap ? mov_v[jf] : 0u;
     const int ntc = (n_cap + kStayTile - 1) >> kStayShift;
-    for (int i = threadIdx.x; i < ntc; i += 256) lt[i] = tsamp[i];
+    {
+        const uint4 *t4 = (const uint4 *)tsamp;
+        uint4 *l4 = (uint4 *)lt;
+        const int nv = (ntc + 3) >> 2;
+        for (int v0 = threadIdx.x; v0 < nv; v0 += 4 * 256) {
+            uint4 c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int v = v0 + u * 256;
+                c[u] = v < nv ? t4[v] : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (v0 + u * 256 < nv) l4[v0 + u * 256] = c[u];
+        }
+    }
     const int s = *n_dev, m = counts[1];
     if (blockIdx.x * 256 >= m) return;   // whole block leaves together
     const int nt = (s + kStayTile - 1) >> kStayShift;   // tiles (samples in tsamp)
