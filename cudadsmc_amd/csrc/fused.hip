@@ -1360,6 +1360,7 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
     __shared__ uint32_t ptmp[256 / kWave + 1];
     __shared__ unsigned long long ptmp64[256 / kWave + 1];
     __shared__ bool fin;
+    __shared__ double part[kMaxFinalizeP * kAccK];   // partial sums of a trap's P blocks
     if ((int)blockIdx.x >= P * T) {
         resort_prep_block((int)blockIdx.x - P * T, *prep.n_dev, prep.blk_moved, prep.blk_dead,
                           prep.per_tile, prep.mask_m, prep.mask_d, prep.pre_m, prep.pre_d,
@@ -1385,18 +1386,19 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
     for (int q = 0; q < kAccK; ++q) a[q] = 0.0;
     uint32_t dead = 0;   // removed atoms before s0 (this thread's share)
     if (trap_next) {
-        // whole move blocks below s0, four counts per 16-byte load
+        // whole move blocks below s0, four counts per 16-byte load, eight
+        // loads per thread in flight
         const int bd = s0 / move_block, nv = bd >> 2;
         const uint4 *v4 = (const uint4 *)blk_dead;
-        for (int v0 = p * 256 + tid; v0 < nv; v0 += 4 * stride) {
-            uint4 c[4];
+        for (int v0 = p * 256 + tid; v0 < nv; v0 += 8 * stride) {
+            uint4 c[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 const int v = v0 + u * stride;
                 c[u] = v < nv ? v4[v] : make_uint4(0u, 0u, 0u, 0u);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) dead += c[u].x + c[u].y + c[u].z + c[u].w;
+            for (int u = 0; u < 8; ++u) dead += c[u].x + c[u].y + c[u].z + c[u].w;
         }
         if (p == 0 && tid < (bd & 3)) dead += blk_dead[4 * nv + tid];
         // then the 64-slot words of the block holding s0, up to s0
@@ -1424,9 +1426,9 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
     block_sum8<256>(a, red);
     // one block per trap, or the last of the trap's P blocks, finalizes it
     double f[kAccK];
-    if (P == 1) {
 #pragma unroll
-        for (int q = 0; q < kAccK; ++q) f[q] = a[q];
+    for (int q = 0; q < kAccK; ++q) f[q] = a[q];
+    if (P == 1) {
         if (tid == 0) fin = true;
     } else if (tid == 0) {
 #pragma unroll
@@ -1435,19 +1437,31 @@ __global__ __launch_bounds__(256) void k_obs_finalize(
                                __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         fin = atomicAdd(&cnt[t], 1u) == (unsigned)(P - 1);
-        if (fin) {
-            __threadfence();
+    }
+    __syncthreads();
+    if (!fin) return;   // whole block leaves together
+    if (P > 1) {
+        // the last of the trap's P blocks to finish: every partial loaded at
+        // once, then thread q sums quantity q in block order (the same order
+        // whichever block finishes last)
+        __threadfence();
+        for (int e = tid; e < P * kAccK; e += 256)
+            part[e] = __hip_atomic_load(&lvl2[(size_t)t * P * kAccK + e], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (tid < kAccK) {
+            double s = 0.0;
+            for (int pp = 0; pp < P; ++pp) s += part[pp * kAccK + tid];
+            part[tid] = s;   // no other thread reads entry tid
+        }
+        __syncthreads();
+        if (tid == 0) {
 #pragma unroll
-            for (int q = 0; q < kAccK; ++q) f[q] = 0.0;
-            for (int pp = 0; pp < P; ++pp)
-                for (int q = 0; q < kAccK; ++q)
-                    f[q] += __hip_atomic_load(&lvl2[((size_t)t * P + pp) * kAccK + q],
-                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int q = 0; q < kAccK; ++q) f[q] = part[q];
             cnt[t] = 0u;
         }
     }
-    __syncthreads();
-    if (!fin || tid != 0) return;
+    if (tid != 0) return;
     for (int q = 0; q < kAccK; ++q) {
         f[q] += over[q];
         acc_over[t * kAccK + q] = 0.0;

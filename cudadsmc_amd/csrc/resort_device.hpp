@@ -65,21 +65,22 @@ __device__ __forceinline__ void resort_prep_block(
     uint64_t mm = 0ull, md = 0ull;
     if (g < words) { mm = mask_m[g]; md = mask_d[g]; }
     // move blocks before this block: 8 k tiles of per_tile each, a multiple
-    // of 8 entries, read as 16-byte vectors, four per thread in flight
+    // of 8 entries, read as 16-byte vectors, eight of each array per thread
+    // in flight
     uint32_t bm = 0, bd = 0;
     {
         const uint4 *vm = (const uint4 *)blk_moved, *vd = (const uint4 *)blk_dead;
         const int nv = k * (kPrepWords / kTileWords) * per_tile / 4;
-        for (int v0 = tid; v0 < nv; v0 += 4 * kPrepWords) {
-            uint4 a[4], d[4];
+        for (int v0 = tid; v0 < nv; v0 += 8 * kPrepWords) {
+            uint4 a[8], d[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 const int v = v0 + u * kPrepWords;
                 a[u] = v < nv ? vm[v] : make_uint4(0, 0, 0, 0);
                 d[u] = v < nv ? vd[v] : make_uint4(0, 0, 0, 0);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 bm += a[u].x + a[u].y + a[u].z + a[u].w;
                 bd += d[u].x + d[u].y + d[u].z + d[u].w;
             }
