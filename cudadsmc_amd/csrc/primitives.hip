@@ -220,8 +220,9 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 //      bucket and wave), or, beyond the room, of an overflow list;
 //   4. merge: one block per tile adds up the bucket counts of the earlier
 //      tiles, ranks its bucket's movers by counting (buckets hold a few
-//      dozen; a bucket beyond its room is gathered with its overflow movers
-//      and sorted in scratch), and writes the tile's stays and movers to
+//      dozen; a bucket beyond the LDS table of kBucketLds movers is gathered
+//      from its room and the overflow list and sorted in scratch), and
+//      writes the tile's stays and movers to
 //      their final positions: with S = stays before the tile plus movers in
 //      earlier buckets, a stay goes to S + (stays of the tile before it) +
 //      (bucket movers with q <= its slot), a mover of rank r to S + (stays
@@ -230,7 +231,7 @@ void radix_sort(const uint32_t *keys, const uint32_t *vals, uint32_t *keys_alt,
 // No kernel waits on another block.
 // ---------------------------------------------------------------------------
 constexpr int kBucketLds = 1024;               // movers a bucket ranks in LDS
-constexpr int kBucketCap = kBucketLds;         // fixed room of a bucket
+constexpr int kBucketCap = 1024;               // fixed room of a bucket (>= kBucketLds)
 constexpr int kBucketBits = 13;
 constexpr int kMaxBuckets = 1 << kBucketBits;  // tiles the locate's LDS sample table holds
 
@@ -536,7 +537,7 @@ __global__ __launch_bounds__(256) void k_ins_merge(
     }
 #pragma unroll
     for (int u = 0; u < kMergePer; ++u) H[tid + 256 * u] = 0u;
-    const bool small = b <= kBucketCap;
+    const bool small = b <= kBucketLds;
     // movers in earlier buckets
     uint32_t before;
     (void)block_exclusive_scan<256>(bsum, tmp, &before);   // syncs: H cleared, stay words in place
@@ -560,10 +561,11 @@ __global__ __launch_bounds__(256) void k_ins_merge(
         // in scratch and sort it there
         uint32_t *ak = srt_k + before, *av = srt_v + before, *aq = srt_q + before;
         uint32_t *ck = alt_k + before, *cv = alt_v + before, *cq = alt_q + before;
-        for (int e = tid; e < kBucketCap; e += 256) {
+        const int nr = min(b, kBucketCap);   // movers in the room
+        for (int e = tid; e < nr; e += 256) {
             ak[e] = fb_k[fb0 + e]; av[e] = fb_v[fb0 + e]; aq[e] = fb_q[fb0 + e];
         }
-        if (tid == 0) n_big = kBucketCap;
+        if (tid == 0) n_big = nr;
         __syncthreads();
         const int no = counts[kOverflowCount];
         for (int e = tid; e < no; e += 256) {

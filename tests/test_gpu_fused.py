@@ -254,12 +254,15 @@ def test_fused_step_equals_split(case, monkeypatch):
     np.testing.assert_allclose(out["1"][5], out["0"][5], rtol=1e-14)
 
 
-def test_incremental_sort_with_crowded_gap():
+@pytest.mark.parametrize("n", [400, 900, 100000])
+def test_incremental_sort_with_crowded_gap(n):
     """Atoms aimed at one cell: on the second step nearly every atom changes
     cell into the same cell, so one gap between two stays receives ~all the
-    movers (the block-level merge sort path of the re-sort), and on the third
-    step they leave it again.  Bitwise equal to the full radix sort."""
-    n, L, nc, dt = 100000, 1e-3, 16, 1e-6
+    movers, and on the third step they leave it again.  n = 400 and 900: the
+    bucket is ranked in the merge's LDS table with hundreds of ties; 1e5: far
+    beyond the bucket's room (room and overflow list gathered and sorted in
+    scratch).  Bitwise equal to the full radix sort."""
+    L, nc, dt = 1e-3, 16, 1e-6
     trap = configs.Trap(kind="box", n_atoms=n, nc=(nc, nc, nc), half_width=(L / 2,) * 3,
                         T=20e-6, mu=0.0, n_real=1e10)
     run = configs.Run(traps=[trap], precision="fp32", dt=dt, seed=9, sigma=0.0,
