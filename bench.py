@@ -11,10 +11,12 @@ Also reported: the FP64 plain (drop-in) form and the FP32 form on the single
 quadrupole trap N = 1e6 / 100^3 cells (the precision-form workload), their
 ratio, the FP64 energy drift of the FP32 runs, a per-kernel roofline from
 hipEvents around every launch of a profiled pass of the same workload, the
-HBM traffic of the dominant kernel from two rocprofv3 counter runs (FETCH_SIZE,
-then WRITE_SIZE) scaled by what the same counters read for a streaming copy of
-known size, and the CPU oracle (OpenMP FP64 restatement of the reference
-kernels) on N = 1e4 / 35^3.
+HBM traffic of each kernel of the step from two rocprofv3 counter runs
+(FETCH_SIZE, then WRITE_SIZE) scaled by what the same counters read for a
+record gather of known size in the move's access pattern, the per-result
+error of the FP32 form against the FP64 form at N = 1e6, the spin and
+evaporation workload at N = 4e6, and the CPU oracle (OpenMP FP64 restatement
+of the reference kernels) on N = 1e4 / 35^3.
 """
 from __future__ import annotations
 
@@ -267,9 +269,12 @@ def pmc_traffic(steps=10, timeout_s=150):
                   gather_fetch=52.0 * GATHER_SLOTS / (f_gat * 1024.0),
                   gather_write=48.0 * GATHER_SLOTS / (w_gat * 1024.0))
     out = dict(scales={k: round(v, 4) for k, v in scales.items()}, steps=steps, kernels={})
-    for key in ("step_fused", "move_fused", "collide_fused", "resort_merge", "resort_locate",
-                "obs_finalize", "collide_edges"):
-        f, w = pick(F, key), pick(W, key)
+    # profiler names -> kernel names
+    names = {"move_fused": "move_fused", "collide_fused": "collide_fused",
+             "resort_merge": "ins_merge", "resort_locate": "ins_locate",
+             "obs_finalize": "obs_finalize", "collide_edges": "collide_edges"}
+    for key, kname in names.items():
+        f, w = pick(F, kname), pick(W, kname)
         if f is None or w is None:
             continue
         fetch = f * 1024.0 * scales["gather_fetch"]
@@ -501,7 +506,8 @@ def main():
         best = min(report, key=lambda r: r["ms_per_step"])
         for r in report:
             log(f"[bench] cells target {r['target_peak']:5.2f}/central cell: {r['cells']} cells, "
-                f"block {r['block']}, {r['ms_per_step'] * 1e3:.1f} us per step")
+                f"{r['traps_unclamped']} traps at the target, block {r['block']}, "
+                f"{r['ms_per_step'] * 1e3:.1f} us per step")
         cells_tuning = dict(best_target_peak=best["target_peak"],
                             candidates=[{k: (round(v, 5) if isinstance(v, float) else v)
                                          for k, v in r.items()} for r in report])

@@ -17,35 +17,64 @@ from __future__ import annotations
 import copy
 from typing import Dict, List, Sequence, Tuple
 
+import math
+
 from .configs import Run, cells_for_trap
 from .simulation import Simulation
 
 # central-cell occupancies tried, a factor sqrt(2) apart
 PEAKS: Tuple[float, ...] = (4.0, 5.66, 8.0, 11.3, 16.0)
 PEAK_RANGE = (4.0, 16.0)
+# cells per dimension of a trap: the batched workload's 32^3..64^3
+NC_RANGE = (32, 64)
 
 
-def regrid(run: Run, target_peak: float) -> Run:
+def central_peak(n_atoms: int, nc: int) -> float:
+    """Test particles in the central cell of a quadrupole trap gridded over
+    +-5 thermal radii with nc^3 cells (the estimate cells_for_trap inverts)."""
+    return 1000.0 * n_atoms / (4.0 * math.pi * nc ** 3)
+
+
+def regrid(run: Run, target_peak: float, nc_range: Tuple[int, int] = NC_RANGE) -> Run:
     """`run` with every trap's grid chosen for `target_peak` test particles in
-    its central cell; everything else (half widths, atoms, schedules) kept."""
+    its central cell, Nc clamped to `nc_range`; everything else (half widths,
+    atoms, schedules) kept."""
     lo, hi = PEAK_RANGE
     if not lo <= target_peak <= hi:
         raise ValueError(f"target_peak {target_peak} outside {lo}..{hi} test particles per central cell")
+    if not 1 <= nc_range[0] <= nc_range[1]:
+        raise ValueError(f"bad cell range {nc_range}")
     out = copy.deepcopy(run)
     for t in out.traps:
         if t.kind != "quadrupole":
             raise ValueError("cells per trap are tuned for quadrupole traps only")
-        nc = cells_for_trap(t.n_atoms, target_peak)
+        nc = cells_for_trap(t.n_atoms, target_peak, lo=nc_range[0], hi=nc_range[1])
         t.nc = (nc, nc, nc)
     return out
 
 
+def grid_record(run: Run, target_peak: float, nc_range: Tuple[int, int] = NC_RANGE) -> Dict:
+    """How a regridded run meets its target: total cells, the traps whose Nc
+    was not clamped, and the central-cell occupancy (min, median, max) over
+    the traps."""
+    import numpy as np
+
+    ncs = [int(t.nc[0]) for t in run.traps]
+    peaks = np.array([central_peak(t.n_atoms, int(t.nc[0])) for t in run.traps])
+    return dict(cells=sum(int(t.nc[0]) * int(t.nc[1]) * int(t.nc[2]) for t in run.traps),
+                traps_unclamped=int(sum(nc_range[0] < n < nc_range[1] for n in ncs)),
+                central_peak=(float(peaks.min()), float(np.median(peaks)), float(peaks.max())))
+
+
 def autotune_cells(run: Run, peaks: Sequence[float] = PEAKS, warm: int = 20,
-                   steps: int = 100, lib=None) -> Tuple[Run, List[Dict]]:
+                   steps: int = 100, lib=None,
+                   nc_range: Tuple[int, int] = NC_RANGE) -> Tuple[Run, List[Dict]]:
     """Time `steps` graph-replayed steps (after `warm` untimed ones and the
-    block-size tuning) of `run` regridded for each target in `peaks`; return
-    the fastest regridded run and one record per candidate
-    {target_peak, cells, block, ms_per_step}."""
+    block-size tuning) of `run` regridded for each target in `peaks` (Nc
+    clamped to `nc_range`); return the fastest regridded run and one record
+    per candidate {target_peak, cells, traps_unclamped, central_peak, block,
+    ms_per_step}: with a narrow range many traps sit at its ends, and the
+    record shows how many reached the target."""
     if run.precision != "fp32":
         raise ValueError("the cells autotuner tunes the batched FP32 Kahan form")
     if not peaks:
@@ -53,7 +82,7 @@ def autotune_cells(run: Run, peaks: Sequence[float] = PEAKS, warm: int = 20,
     report: List[Dict] = []
     best = None
     for p in peaks:
-        cand = regrid(run, p)
+        cand = regrid(run, p, nc_range)
         with Simulation(cand, lib=lib) as sim:
             sim.init_distribution()
             sim.step(4)
@@ -61,8 +90,8 @@ def autotune_cells(run: Run, peaks: Sequence[float] = PEAKS, warm: int = 20,
             sim.step(warm)
             sim.sync()
             ms = sim.time_steps(steps) / steps
-        cells = sum(int(t.nc[0]) * int(t.nc[1]) * int(t.nc[2]) for t in cand.traps)
-        report.append(dict(target_peak=float(p), cells=cells, block=int(block), ms_per_step=ms))
+        report.append(dict(target_peak=float(p), **grid_record(cand, p, nc_range), block=int(block),
+                           ms_per_step=ms))
         if best is None or ms < best[0]:
             best = (ms, cand)
     return best[1], report

@@ -32,6 +32,30 @@ def test_regrid_hits_the_target_occupancy():
     assert [t.nc for t in tune.regrid(run, 8.0).traps] == [t.nc for t in run.traps]
 
 
+def test_grid_record_reports_clamped_traps():
+    """The tuner's record says how many traps reached the target: with the
+    workload's 32..64 range many traps sit at its ends; with a range wide
+    enough for every trap none does, and every central cell is within one
+    grid step of the target."""
+    run = configs.batched_traps(n_traps=64, steps=10)
+    r = tune.regrid(run, 8.0)
+    rec = tune.grid_record(r, 8.0)
+    ncs = [t.nc[0] for t in r.traps]
+    assert rec["traps_unclamped"] == sum(32 < n < 64 for n in ncs)
+    assert rec["traps_unclamped"] < len(ncs)
+    assert rec["cells"] == sum(n ** 3 for n in ncs)
+    lo, med, hi = rec["central_peak"]
+    assert lo <= med <= hi
+    wide = tune.regrid(run, 8.0, nc_range=(4, 256))
+    rec_w = tune.grid_record(wide, 8.0, nc_range=(4, 256))
+    assert rec_w["traps_unclamped"] == len(ncs)
+    for t in wide.traps:
+        n = t.nc[0]
+        assert _occupancy(t.n_atoms, n + 1) <= 8.0 <= _occupancy(t.n_atoms, n - 1)
+    with pytest.raises(ValueError):
+        tune.regrid(run, 8.0, nc_range=(64, 32))
+
+
 def test_regrid_rejects_what_it_does_not_tune():
     run = configs.batched_traps(n_traps=4, steps=10)
     with pytest.raises(ValueError):
