@@ -96,6 +96,27 @@ def run_gpu_leg(run, steps, warmup, autotune=True, repeats=3):
     return sim, dict(med, block=bs, repeats=[round(r["value"], 1) for r in reps])
 
 
+def run_group_leg(runs, steps, warmup, chunk=50, repeats=3):
+    """run_gpu_leg for a group of contexts (one per group of traps, each on
+    its own stream), stepped in turns of `chunk` steps (dsmc_step_group) and
+    timed from one start event to the last stream's end."""
+    from cudadsmc_amd import SimulationGroup
+
+    grp = SimulationGroup(runs)
+    grp.init_distribution()
+    grp.step(4)
+    blocks = grp.autotune()
+    grp.step(warmup)
+    grp.sync()
+    reps = []
+    for _ in range(repeats):
+        ms = grp.time_steps(steps, chunk)
+        n = sum(atom_steps(s, steps) for s in grp.sims)
+        reps.append(dict(ms=ms, atom_steps=n, value=n / (ms * 1e-3)))
+    med = sorted(reps, key=lambda r: r["value"])[len(reps) // 2]
+    return grp, dict(med, blocks=blocks, repeats=[round(r["value"], 1) for r in reps])
+
+
 def energy_drift(sim, steps):
     """max over traps of max_t |E(t) - E(0)| / E(0), E = alive energy plus the
     energy carried away by the atoms removed so far."""
@@ -388,6 +409,9 @@ def main():
                     help="only run the streaming-copy and record-gather calibration kernels")
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the rocprofv3 counter runs (roofline.traffic = null)")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="trap groups of the batched workload, one context and stream each, "
+                         "stepped together (1: one context)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the step's kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--tune-cells", action="store_true",
@@ -431,11 +455,25 @@ def main():
         log(f"[bench] batched: {len(batched.traps)} traps, {batched.n_atoms} atoms")
         sim, b = run_gpu_leg(batched, args.steps, args.warmup)
         b["energy_drift"] = energy_drift(sim, args.steps)
-        log(f"[bench] batched FP32: {b['value']:.4e} atom-steps/s (median of {b['repeats']}; "
-            f"{b['ms']:.1f} ms, block {b['block']}), max |dE/E| {b['energy_drift']:.2e}")
-        result["batched"] = b
+        log(f"[bench] batched FP32, one context: {b['value']:.4e} atom-steps/s (median of "
+            f"{b['repeats']}; {b['ms']:.1f} ms, block {b['block']}), max |dE/E| {b['energy_drift']:.2e}")
+        # the per-kernel pass runs on the one-context run: each kernel alone
         kernels, n_mean = profile_leg(sim, args.profile_steps)
         sim.close()
+        if args.streams > 1 and not args.no_graph:
+            one = b
+            grp, b = run_group_leg(configs.split_traps(batched, args.streams), args.steps,
+                                   args.warmup)
+            lg = grp.obs_log()[-args.steps:]
+            E = lg["ek"] + lg["ep"] + np.cumsum(lg["e_evap"], axis=0)
+            b["energy_drift"] = float(np.max(np.abs(E - E[0]) / np.abs(E[0])))
+            grp.close()
+            b["one_context"] = dict(value=one["value"], ms=one["ms"], repeats=one["repeats"],
+                                    block=one["block"])
+            log(f"[bench] batched FP32, {args.streams} trap groups on {args.streams} streams: "
+                f"{b['value']:.4e} atom-steps/s (median of {b['repeats']}; {b['ms']:.1f} ms), "
+                f"max |dE/E| {b['energy_drift']:.2e}")
+        result["batched"] = b
         dom = max((k for k in kernels if "alg_bytes" in kernels[k]),
                   key=lambda k: kernels[k]["total_ms"])
         d = kernels[dom]
@@ -537,6 +575,8 @@ def main():
             "dt": 1e-6,
             "precision": "fp32-kahan, fp64 reductions",
             "parallelism": "single-gpu",
+            "trap_groups": (args.streams if "batched" in result and "one_context" in result["batched"]
+                            else 1),
         },
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -7,6 +7,7 @@ mirror of the reference driver's call sequence.
 from . import _abi, configs
 from ._abi import DsmcError, load_library
 from .configs import Run, Trap
-from .simulation import OBS_DTYPE, Simulation
+from .simulation import OBS_DTYPE, Simulation, SimulationGroup
 
-__all__ = ["DsmcError", "load_library", "Run", "Trap", "Simulation", "OBS_DTYPE", "configs", "_abi"]
+__all__ = ["DsmcError", "load_library", "Run", "Trap", "Simulation", "SimulationGroup", "OBS_DTYPE",
+           "configs", "_abi"]

@@ -105,6 +105,8 @@ PROTOTYPES = {
     "dsmc_profile_enable": (I, [P, I]),
     "dsmc_profile_read": (I, [P, C.POINTER(KernelStat), I, C.POINTER(I)]),
     "dsmc_autotune": (I, [P, C.POINTER(I)]),
+    "dsmc_step_group": (I, [C.POINTER(P), I, I, I]),
+    "dsmc_time_steps_group": (I, [C.POINTER(P), I, I, I, C.POINTER(C.c_double)]),
     "dsmc_sort_by_key": (I, [P, P, I, I, P]),
     "dsmc_sort_by_key_noalloc": (I, [P, P, I, I, P]),
     "dsmc_exclusive_scan": (I, [P, P, I, P]),

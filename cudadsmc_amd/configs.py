@@ -202,6 +202,32 @@ def cells_for_trap(n_atoms: int, target_peak: float = 8.0, lo: int = 32, hi: int
     return int(min(hi, max(lo, round(nc))))
 
 
+def split_traps(run: Run, groups: int) -> List[Run]:
+    """`run` as `groups` runs of consecutive traps with about equal atom
+    counts, for stepping on one GPU as a group of contexts.  Every trap keeps
+    its Philox stream (its index in `run` plus its philox_stream), so its
+    results do not change."""
+    import numpy as np
+
+    T = len(run.traps)
+    if not 1 <= groups <= T:
+        raise ValueError(f"{groups} groups for {T} traps")
+    n = np.array([t.n_atoms for t in run.traps], dtype=np.float64)
+    cum = np.cumsum(n)
+    cuts = [0]
+    for g in range(1, groups):
+        c = int(np.searchsorted(cum, cum[-1] * g / groups)) + 1
+        cuts.append(min(max(c, cuts[-1] + 1), T - (groups - g)))
+    cuts.append(T)
+    out = []
+    for a, b in zip(cuts, cuts[1:]):
+        traps = [copy.deepcopy(t) for t in run.traps[a:b]]
+        for t in traps:
+            t.philox_stream = int(t.philox_stream) + a
+        out.append(run.with_(traps=traps, name=f"{run.name}[{a}:{b}]"))
+    return out
+
+
 def batched_traps(n_traps: int = 256, seed: int = 1, steps: int = 5000) -> Run:
     import numpy as np
 

@@ -1600,6 +1600,74 @@ dsmc_status dsmc_time_steps(dsmc_ctx *ctx, int n_steps, double *ms)
     DSMC_API_END(ctx)
 }
 
+// ---- several contexts stepped together ------------------------------------
+// Independent groups of traps in separate contexts (each with its own stream
+// and graph) are stepped in turns of `chunk` steps, so that every stream has
+// work queued and the GPU runs one group's latency-bound kernels (re-sort,
+// collide, finalize) beside another's bandwidth-bound move.
+static void check_group(dsmc_ctx *const *ctxs, int n_ctx, int n_steps, int chunk)
+{
+    if (n_steps < 0 || chunk < 1) fail(DSMC_EINVAL, "n_steps < 0 or chunk < 1");
+    for (int i = 0; i < n_ctx; ++i) {
+        if (!ctxs[i]) fail(DSMC_EINVAL, "null context in the group");
+        if (ctxs[i]->device != ctxs[0]->device) fail(DSMC_EINVAL, "the group's contexts are on different devices");
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) fail(DSMC_EINVAL, "a context appears twice in the group");
+    }
+}
+
+static void group_steps(dsmc_ctx *const *ctxs, int n_ctx, int n_steps, int chunk)
+{
+    for (int done = 0; done < n_steps; done += chunk) {
+        const int k = std::min(chunk, n_steps - done);
+        for (int i = 0; i < n_ctx; ++i) do_steps(ctxs[i], k);
+    }
+}
+
+dsmc_status dsmc_step_group(dsmc_ctx *const *ctxs, int n_ctx, int n_steps, int chunk)
+{
+    if (!ctxs || n_ctx < 1) return DSMC_EINVAL;
+    DSMC_API_BEGIN(ctxs[0])
+    check_group(ctxs, n_ctx, n_steps, chunk);
+    group_steps(ctxs, n_ctx, n_steps, chunk);
+    DSMC_API_END(ctxs[0])
+}
+
+dsmc_status dsmc_time_steps_group(dsmc_ctx *const *ctxs, int n_ctx, int n_steps, int chunk, double *ms)
+{
+    if (!ctxs || n_ctx < 1) return DSMC_EINVAL;
+    DSMC_API_BEGIN(ctxs[0])
+    if (!ms) fail(DSMC_EINVAL, "null output");
+    check_group(ctxs, n_ctx, n_steps, chunk);
+    std::vector<hipEvent_t> ev((size_t)n_ctx + 1, nullptr);
+    auto destroy = [&]() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+    try {
+        for (auto &e : ev) DSMC_HIP(hipEventCreate(&e));
+        for (int i = 0; i < n_ctx; ++i) DSMC_HIP(hipStreamSynchronize(ctxs[i]->stream));
+        // one start for all: every other stream waits for the first's event
+        DSMC_HIP(hipEventRecord(ev[0], ctxs[0]->stream));
+        for (int i = 1; i < n_ctx; ++i) DSMC_HIP(hipStreamWaitEvent(ctxs[i]->stream, ev[0], 0));
+        group_steps(ctxs, n_ctx, n_steps, chunk);
+        for (int i = 0; i < n_ctx; ++i) DSMC_HIP(hipEventRecord(ev[i + 1], ctxs[i]->stream));
+        float worst = 0.f;
+        for (int i = 0; i < n_ctx; ++i) {
+            DSMC_HIP(hipEventSynchronize(ev[i + 1]));
+            float f = 0.f;
+            DSMC_HIP(hipEventElapsedTime(&f, ev[0], ev[i + 1]));
+            worst = std::max(worst, f);
+        }
+        *ms = worst;
+    } catch (...) {
+        destroy();
+        throw;
+    }
+    destroy();
+    DSMC_API_END(ctxs[0])
+}
+
 dsmc_status dsmc_profile_enable(dsmc_ctx *ctx, int enable)
 {
     DSMC_API_BEGIN(ctx)

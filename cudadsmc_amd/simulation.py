@@ -206,3 +206,58 @@ class Simulation:
 OBS_DTYPE = np.dtype([("ek", "f8"), ("ep", "f8"), ("v2", "f8", 3), ("e_evap", "f8"),
                       ("n_alive", "i8"), ("n_evap", "i8"), ("collisions", "i8"), ("step", "i8")])
 assert OBS_DTYPE.itemsize == C.sizeof(_abi.Observables)
+
+
+class SimulationGroup:
+    """Several contexts stepped together, one per group of independent traps
+    (`configs.split_traps`): each context has its own stream and graph, and
+    `step` submits their steps in turns (dsmc_step_group) so that the groups
+    overlap on the GPU.  A trap's results are those of the same trap in one
+    context holding all of them (its Philox stream is kept)."""
+
+    def __init__(self, runs: List[Run], lib=None):
+        if not runs:
+            raise ValueError("an empty group")
+        self.lib = lib or _abi.load_library()
+        self.sims = [Simulation(r, lib=self.lib) for r in runs]
+
+    def _handles(self):
+        hs = [s.h for s in self.sims]
+        return (C.c_void_p * len(hs))(*hs)
+
+    def close(self):
+        for s in self.sims:
+            s.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def init_distribution(self):
+        for s in self.sims:
+            s.init_distribution()
+
+    def autotune(self) -> List[int]:
+        return [s.autotune() for s in self.sims]
+
+    def step(self, n_steps: int = 1, chunk: int = 50):
+        _abi.check(self.lib, self.lib.dsmc_step_group(self._handles(), len(self.sims), int(n_steps),
+                                                      int(chunk)), self.sims[0].h)
+
+    def time_steps(self, n_steps: int, chunk: int = 50) -> float:
+        ms = C.c_double()
+        _abi.check(self.lib, self.lib.dsmc_time_steps_group(self._handles(), len(self.sims), int(n_steps),
+                                                            int(chunk), C.byref(ms)), self.sims[0].h)
+        return ms.value
+
+    def sync(self):
+        for s in self.sims:
+            s.sync()
+
+    def obs_log(self) -> np.ndarray:
+        """The groups' observable logs side by side: (steps, traps) in the
+        order of the traps before the split."""
+        return np.concatenate([s.obs_log() for s in self.sims], axis=1)
+

@@ -230,6 +230,13 @@ dsmc_status dsmc_compact(dsmc_ctx *ctx, int *n_alive);
 dsmc_status dsmc_step(dsmc_ctx *ctx, int n_steps);
 /* Block until all work on the context's stream is done. */
 dsmc_status dsmc_sync(dsmc_ctx *ctx);
+/* dsmc_step on n_ctx contexts of one device, n_steps each, submitted in
+ * turns of `chunk` steps per context; each context's work stays on its own
+ * stream, so independent groups of traps (one context each) overlap on the
+ * GPU.  Asynchronous like dsmc_step; errors are reported on ctxs[0].  No
+ * counterpart in the reference (it steps one trap); for callers that shard a
+ * batch of independent traps over several contexts on one GPU. */
+dsmc_status dsmc_step_group(dsmc_ctx *const *ctxs, int n_ctx, int n_steps, int chunk);
 
 /* ---- output surface (main.cu:526-579) --------------------------------- */
 
@@ -246,6 +253,11 @@ dsmc_status dsmc_field_bytes(dsmc_ctx *ctx, int field, size_t *bytes);
 /* Run n_steps motion steps between two hipEvents recorded on the context's
  * stream; *ms receives the elapsed time. */
 dsmc_status dsmc_time_steps(dsmc_ctx *ctx, int n_steps, double *ms);
+/* dsmc_step_group between a hipEvent recorded on ctxs[0]'s stream (every
+ * other stream waits for it) and the last stream's end; *ms receives the
+ * elapsed time.  Synchronises. */
+dsmc_status dsmc_time_steps_group(dsmc_ctx *const *ctxs, int n_ctx, int n_steps, int chunk,
+                                  double *ms);
 /* Per-kernel timing: when enabled, every kernel launch is bracketed by
  * hipEvents on the context's stream and launched eagerly (no graph). */
 dsmc_status dsmc_profile_enable(dsmc_ctx *ctx, int enable);
