@@ -584,6 +584,8 @@ def main():
     if "batched" in result:
         line["repeats"] = result["batched"]["repeats"]
         line["energy_drift"] = result["batched"]["energy_drift"]
+        if "one_context" in result["batched"]:
+            line["one_context"] = result["batched"]["one_context"]
         line["step_roofline"] = step_roof
     if "drift" in result:
         line["energy_drift_1e4"] = result["drift"]

@@ -375,7 +375,8 @@ def test_reads_after_a_removing_step():
     (the log sums the move's FP32 per-atom squares and potentials, dsmc_observe
     FP64 ones of value + remainder: within 1e-6), its alive counts equal the
     trap ranges of copy_out, and the survivors' velocities from copy_out give
-    its sums; no survivor lies above the cut."""
+    its sums; no survivor lies above the cut.  dsmc_observe and
+    dsmc_calculate_temp give the same bits on repeated calls."""
     traps = []
     for n, dBdz, T in [(3000, 1.5, 30e-6), (20000, 4.0, 8e-6), (7000, 2.5, 15e-6), (12000, 3.0, 20e-6)]:
         t = configs.quadrupole_trap(n, configs.cells_for_trap(n), T=T, dBdz=dBdz, n_real=3e6)
@@ -391,6 +392,12 @@ def test_reads_after_a_removing_step():
             sim.step(steps)
             last = sim.obs_log()[-1]
             obs = sim.observe()
+            # fixed-order sums: the same bits on every call
+            again = sim.observe()
+            for a, b in zip(obs, again):
+                assert (a.ek, a.ep, tuple(a.v2), a.n_alive) == (b.ek, b.ep, tuple(b.v2), b.n_alive)
+            T1, T2 = sim.calculate_temp(), sim.calculate_temp()
+            assert T1 == T2 and T1 > 0
             ts = sim.copy_out(_abi.FIELD_TRAP_START)
             vel = sim.copy_out(_abi.FIELD_VEL)
             pos = sim.copy_out(_abi.FIELD_POS)
