@@ -21,6 +21,7 @@
 // so the loop can be captured once and replayed as a HIP graph.
 #include "fused.hpp"
 #include "resort_device.hpp"
+#include "sincos_f32.hpp"
 
 #include <algorithm>
 #include <cstdlib>
@@ -743,6 +744,23 @@ __device__ __forceinline__ void move_load(const MoveParams &mp, const FusedPtrs 
     m.kold = 0;
 }
 
+// The Bloch vector s rotated by the angle phi about the unit axis n
+// (Rodrigues): s cos phi + (n x s) sin phi + n (n . s)(1 - cos phi); the
+// precession of one spin substep, phi = gs muB |B| dt / hbar.
+__device__ __forceinline__ void rotate_about(const F3 &n, float phi, float &sx, float &sy, float &sz)
+{
+    float sn, cs;
+    sincos_f32(phi, &sn, &cs);
+    const float nd = fmaf(n.x, sx, fmaf(n.y, sy, n.z * sz));
+    const float kx = fmaf(n.y, sz, -(n.z * sy));
+    const float ky = fmaf(n.z, sx, -(n.x * sz));
+    const float kz = fmaf(n.x, sy, -(n.y * sx));
+    const float om = nd * (1.0f - cs);
+    sx = fmaf(sx, cs, fmaf(kx, sn, n.x * om));
+    sy = fmaf(sy, cs, fmaf(ky, sn, n.y * om));
+    sz = fmaf(sz, cs, fmaf(kz, sn, n.z * om));
+}
+
 // Spin substeps, Kahan velocity Verlet, wrap, evaporation test, next key;
 // stores the atom into slot i of B.  vals = (vx^2, vy^2, vz^2, U) of a
 // survivor; lost_e = energy of a removed atom.
@@ -766,21 +784,27 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
         const float w = (float)(mp.gs * kMuB * dts / kHbar);
         const float fdts = (float)dts;
         float sx = m.sx, sy = m.sy, sz = m.sz;
-        for (int k = 0; k < mp.substeps; ++k) {
-            const float tau = mp.interp ? ((float)k + 0.5f) * fdts : 0.0f;
-            F3 Bn;
-            const float Bm = field_dir_f(tr, m.f.px + m.f.vx * tau, m.f.py + m.f.vy * tau,
-                                         m.f.pz + m.f.vz * tau, Bn);
-            float sn, cs;
-            sincosf(w * Bm, &sn, &cs);
-            const float nd = Bn.x * sx + Bn.y * sy + Bn.z * sz;
-            const float kx = Bn.y * sz - Bn.z * sy;
-            const float ky = Bn.z * sx - Bn.x * sz;
-            const float kz = Bn.x * sy - Bn.y * sx;
-            const float om = nd * (1.0f - cs);
-            sx = sx * cs + kx * sn + Bn.x * om;
-            sy = sy * cs + ky * sn + Bn.y * om;
-            sz = sz * cs + kz * sn + Bn.z * om;
+        if (tr.kind == kTrapQuad) {
+            // the quadrupole field inline: B ~ (x, y, -2 z), |B| = B' rho
+            const float dBdz = tr.f_dBdz;
+            for (int k = 0; k < mp.substeps; ++k) {
+                const float tau = mp.interp ? ((float)k + 0.5f) * fdts : 0.0f;
+                const float x = fmaf(m.f.vx, tau, m.f.px), y = fmaf(m.f.vy, tau, m.f.py),
+                            z = fmaf(m.f.vz, tau, m.f.pz);
+                const float r2 = fmaf(x, x, fmaf(y, y, 4.0f * (z * z)));
+                const bool zero = !(r2 > 0.0f);   // on the axis' zero: no rotation
+                const float inv = zero ? 0.0f : rsqrt_f(r2);
+                const F3 Bn = { x * inv, y * inv, zero ? 1.0f : (-2.0f * z) * inv };
+                rotate_about(Bn, w * (dBdz * (r2 * inv)), sx, sy, sz);
+            }
+        } else {
+            for (int k = 0; k < mp.substeps; ++k) {
+                const float tau = mp.interp ? ((float)k + 0.5f) * fdts : 0.0f;
+                F3 Bn;
+                const float Bm = field_dir_f(tr, fmaf(m.f.vx, tau, m.f.px), fmaf(m.f.vy, tau, m.f.py),
+                                             fmaf(m.f.vz, tau, m.f.pz), Bn);
+                rotate_about(Bn, w * Bm, sx, sy, sz);
+            }
         }
         const double L = sqrt((double)sx * sx + (double)sy * sy + (double)sz * sz);
         m.sx = (float)((double)sx / L); m.sy = (float)((double)sy / L); m.sz = (float)((double)sz / L);
