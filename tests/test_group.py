@@ -2,6 +2,8 @@
 (configs.split_traps, SimulationGroup, dsmc_step_group): the split on the
 CPU; on the GPU, every trap of the group bitwise equal to the same trap in
 one context holding all of them."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -64,3 +66,24 @@ def test_group_equals_one_context():
         t0 += nt
         label0 += p.n_atoms
     assert int(lg1["n_evap"][1:].sum()) > 0   # the cuts removed atoms on the way
+
+
+@pytest.mark.gpu
+def test_group_rejects_bad_arguments():
+    run = configs.batched_traps(n_traps=4, steps=10).with_(obs_capacity=64)
+    a, b = configs.split_traps(run, 2)
+    print("group argument checks", flush=True)
+    with SimulationGroup([a, b]) as grp:
+        grp.init_distribution()
+        lib = grp.lib
+        h = grp.sims[0].h
+        twice = (C.c_void_p * 2)(h, h)
+        assert lib.dsmc_step_group(twice, 2, 4, 2) == _abi.DSMC_EINVAL
+        assert "twice" in lib.dsmc_last_error(h).decode()
+        one = (C.c_void_p * 1)(h)
+        assert lib.dsmc_step_group(one, 1, 4, 0) == _abi.DSMC_EINVAL
+        assert lib.dsmc_step_group(one, 0, 4, 2) == _abi.DSMC_EINVAL
+        assert lib.dsmc_step_group(None, 1, 4, 2) == _abi.DSMC_EINVAL
+        grp.step(4, chunk=3)   # a chunk that does not divide the steps
+        grp.sync()
+        assert all(s.info().step == 4 for s in grp.sims)
