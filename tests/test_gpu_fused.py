@@ -99,11 +99,26 @@ def test_move_keys_equal_fp64_index(orc):
     assert int(np.sum(expect == orc.ncells(p))) > 0   # atoms outside the grid
 
 
-def test_fused_motion_matches_fp64(orc):
-    """Collision-free motion (sigma = 0): FP32 Kahan vs the FP64 path."""
-    run = quad(50000, 40, sigma=0.0)
+# tolerances (position over the thermal radius for the quadrupole, over the
+# cloud's extent otherwise; velocity over the largest speed) after 2000
+# collision-free steps from the same state
+MOTION_TOL = {"quadrupole": (1e-6, 1e-5), "ip": (1e-7, 1e-6), "box": (1e-6, 1e-12)}
+
+
+@pytest.mark.parametrize("kind", ["quadrupole", "ip", "box"])
+def test_fused_motion_matches_fp64(orc, kind):
+    """Collision-free motion (sigma = 0): FP32 Kahan vs the FP64 path, in
+    each trap kind: the quadrupole, the Ioffe-Pritchard field of the
+    reference (moveAtoms.cu:93-157) and the periodic box."""
+    if kind == "quadrupole":
+        run = quad(50000, 40, sigma=0.0)
+    elif kind == "ip":
+        run = configs.reference(Nc=40, N=50000).with_(
+            precision="fp32", sigma=0.0, spin=False, spin_loss=False, mcwf=False, lpc=1, dt=1e-6)
+    else:
+        run = configs.box_thermalise("fp32", n_atoms=50000, nc=32).with_(sigma=0.0)
     ref = orc.Reference(run.with_(precision="fp64"))
-    print("fused motion vs fp64, 2000 steps", flush=True)
+    print(f"fused motion vs fp64, {kind}, 2000 steps", flush=True)
     with Simulation(run) as s32, Simulation(run.with_(precision="fp64")) as s64:
         s32.set_state(ref.s["pos"], ref.s["vel"])
         s64.set_state(ref.s["pos"], ref.s["vel"])
@@ -111,11 +126,13 @@ def test_fused_motion_matches_fp64(orc):
         s64.step(2000)
         p32, p64 = by_label(s32, _abi.FIELD_POS), by_label(s64, _abi.FIELD_POS)
         v32, v64 = by_label(s32, _abi.FIELD_VEL), by_label(s64, _abi.FIELD_VEL)
-    rth = run.traps[0].thermal_radius()
-    dx = np.abs(p32 - p64).max() / rth
+    # the quadrupole over its thermal radius; the others over the cloud
+    scale = run.traps[0].thermal_radius() if kind == "quadrupole" else np.abs(p64).max()
+    dx = np.abs(p32 - p64).max() / scale
     dv = np.abs(v32 - v64).max() / np.abs(v64).max()
-    print(f"max |dx|/r_th = {dx:.3e}, max |dv|/v = {dv:.3e}", flush=True)
-    assert dx < 1e-6 and dv < 1e-5
+    print(f"{kind}: max |dx| / scale = {dx:.3e}, max |dv| / v = {dv:.3e}", flush=True)
+    tx, tv = MOTION_TOL[kind]
+    assert dx < tx and dv < tv
 
 
 @pytest.mark.parametrize("n,nc,steps", [(1000000, 100, 10000)])
