@@ -792,10 +792,11 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
                 const float x = fmaf(m.f.vx, tau, m.f.px), y = fmaf(m.f.vy, tau, m.f.py),
                             z = fmaf(m.f.vz, tau, m.f.pz);
                 const float r2 = fmaf(x, x, fmaf(y, y, 4.0f * (z * z)));
-                const bool zero = !(r2 > 0.0f);   // on the axis' zero: no rotation
+                // at the field's zero (or a NaN position) no rotation, as field_dir_f
+                const bool zero = !(r2 > 0.0f);
                 const float inv = zero ? 0.0f : rsqrt_f(r2);
                 const F3 Bn = { x * inv, y * inv, zero ? 1.0f : (-2.0f * z) * inv };
-                rotate_about(Bn, w * (dBdz * (r2 * inv)), sx, sy, sz);
+                rotate_about(Bn, zero ? 0.0f : w * (dBdz * (r2 * inv)), sx, sy, sz);
             }
         } else {
             for (int k = 0; k < mp.substeps; ++k) {
