@@ -57,9 +57,10 @@ ALG_BYTES = {
 }
 SPIN_BYTES = 24.0
 # Streams the fused move (and the fused step) carry beyond that state: the
-# sort permutation (4), the atom label read and written (8), the last sort's
-# key of the slot (4) and the next cell key (4).
-FUSED_EXTRA_BYTES = 20.0
+# sort permutation (4), the last sort's key of the slot (4) and the next cell
+# key (4).  The atom label travels inside the state records (in the low bits
+# of the position remainders), so it adds no bytes.
+FUSED_EXTRA_BYTES = 12.0
 
 
 def log(msg):

@@ -411,7 +411,6 @@ static void build_context(dsmc_ctx *x, const dsmc_config *c, const dsmc_trap *tr
         for (int b = 0; b < 2; ++b) {
             x->fs[b].r = x->alloc<f4>(rec_elems(N));
             for (int k = 0; k < 3; ++k) x->fs[b].s[k] = c->spin ? x->alloc<float>(N) : nullptr;
-            x->fs[b].id = x->alloc<int>(N);
         }
         resort_scratch_bind(x->res, N, x->alloc<char>(resort_scratch_bytes(N), false));
         // one entry per block of the smallest move block size, whole 2048-slot
@@ -1071,17 +1070,21 @@ static void copy_field(dsmc_ctx *x, int field, void *host, size_t bytes)
                 cols[c].resize(N);
                 for (size_t i = 0; i < N; ++i) {
                     const f4 *q = rec.data() + rec_index(live[i]);
-                    const float a = q[h[0] * kRecTile][h[1]], b = q[l[0] * kRecTile][l[1]];
+                    const float a = q[h[0] * kRecTile][h[1]];
+                    const float b = pos ? rem_value(q[l[0] * kRecTile][l[1]]) : q[l[0] * kRecTile][l[1]];
                     cols[c][i] = comp ? (double)b : (double)a + (double)b;
                 }
             }
             interleave(hd, cols, 3, N);
             return;
         }
-        case DSMC_FIELD_ATOM_ID: {
-            const std::vector<int> a = d2h(s.id, (size_t)n_slots);
+        case DSMC_FIELD_ATOM_ID: {   // the labels travel in the position remainders
+            const std::vector<f4> rec = d2h(s.r, rec_elems((size_t)n_slots));
             int *hi_ = (int *)host;
-            for (size_t i = 0; i < N; ++i) hi_[i] = a[live[i]];
+            for (size_t i = 0; i < N; ++i) {
+                const f4 *q = rec.data() + rec_index(live[i]);
+                hi_[i] = rec_label(q[0][3], q[2 * kRecTile][0], q[2 * kRecTile][1]);
+            }
             return;
         }
         case DSMC_FIELD_SPIN:
@@ -1337,15 +1340,15 @@ dsmc_status dsmc_set_state(dsmc_ctx *ctx, const double *pos, const double *vel, 
                 for (size_t i = 0; i < N; ++i) {
                     f4 *q = rec.data() + rec_index((uint32_t)i);
                     const double v = src[3 * i + c];
-                    const float hv = (float)v;
+                    const float hv = (float)v, lv = (float)(v - (double)hv);
                     q[h[0] * kRecTile][h[1]] = hv;
-                    q[l[0] * kRecTile][l[1]] = (float)(v - (double)hv);
+                    // slot i holds atom i: its label in the position remainders
+                    q[l[0] * kRecTile][l[1]] = which ? lv : rem_tag(lv, label_part((int)i, c));
                 }
             }
         }
         DSMC_HIP(hipMemcpyAsync(s.r, rec.data(), rec.size() * sizeof(f4), hipMemcpyHostToDevice,
                                 ctx->stream));
-        DSMC_HIP(hipMemcpyAsync(s.id, ids.data(), N * 4, hipMemcpyHostToDevice, ctx->stream));
         fused_index(ctx);
     }
     DSMC_HIP(hipStreamSynchronize(ctx->stream));

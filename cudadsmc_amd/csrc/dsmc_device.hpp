@@ -328,6 +328,12 @@ __device__ __forceinline__ float field_dir_f(const TrapDev &t, float x, float y,
     return 0.0f;
 }
 
+// x^2 + y^2 + 4 z^2 of the quadrupole field's magnitude (B' sqrt of it)
+__device__ __forceinline__ float quad_r2(float x, float y, float z)
+{
+    return fmaf(x, x, fmaf(y, y, 4.0f * (z * z)));
+}
+
 __device__ __forceinline__ F3 accel_f(const TrapDev &t, float x, float y, float z)
 {
     F3 a = { 0.0f, 0.0f, 0.0f };
@@ -342,7 +348,7 @@ __device__ __forceinline__ F3 accel_f(const TrapDev &t, float x, float y, float 
         float p = t.f_neg_mu_over_m;
         a.x = p * gx; a.y = p * gy; a.z = p * gz;
     } else if (t.kind == kTrapQuad) {
-        const float r2 = x * x + y * y + 4.0f * (z * z);
+        const float r2 = quad_r2(x, y, z);
         if (r2 > 0.0f) {
             const float g = t.f_neg_mu_over_m * t.f_dBdz * rsqrt_f(r2);
             a.x = g * x; a.y = g * y; a.z = g * (4.0f * z);

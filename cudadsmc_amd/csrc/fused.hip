@@ -63,11 +63,12 @@ __device__ __forceinline__ void load_table(int *lds, const int *src, int cnt)
     __syncthreads();
 }
 
-// s + d with the running remainder c: y = d + c, then an error-free TwoSum of
-// s + y (Knuth), so the remainder stays exact whatever the magnitudes.
-__device__ __forceinline__ void kahan_add(float &s, float &c, float d)
+// s + a b with the running remainder c: y = a b + c in one rounding (fma),
+// then an error-free TwoSum of s + y (Knuth), so the remainder stays exact
+// whatever the magnitudes.
+__device__ __forceinline__ void kahan_fma(float &s, float &c, float a, float b)
 {
-    const float y = d + c;
+    const float y = fmaf(a, b, c);
     const float t = s + y;
     const float bp = t - s;
     const float ap = t - bp;
@@ -104,13 +105,12 @@ __global__ __launch_bounds__(kFBlock) void k_init_fused(const TrapDev *__restric
     split_df(vx, st.vx, st.rvx);
     split_df(vy, st.vy, st.rvy);
     split_df(vz, st.vz, st.rvz);
-    rec_store(s.r, (uint32_t)a, st);
+    rec_store(s.r, (uint32_t)a, st, a);
     if (ip.spin) {
         D3 Bn;
         (void)field_dir(tr, px, py, pz, Bn);
         s.s[0][a] = (float)Bn.x; s.s[1][a] = (float)Bn.y; s.s[2][a] = (float)Bn.z;
     }
-    s.id[a] = a;
 }
 
 // indexAtoms for the FP32 state: the FP64 index of p + cp.  With keep_dead, a
@@ -132,9 +132,9 @@ __global__ __launch_bounds__(kFBlock) void k_index_fused(const TrapDev *__restri
     const TrapDev &tr = traps[t];
     const f4 *q = s.r + rec_index((uint32_t)i);
     const f4 p0 = q[0], p2 = q[2 * kRecTile];
-    const double X = (double)p0.x + (double)p0.w;
-    const double Y = (double)p0.y + (double)p2.x;
-    const double Z = (double)p0.z + (double)p2.y;
+    const double X = (double)p0.x + (double)rem_value(p0.w);
+    const double Y = (double)p0.y + (double)rem_value(p2.x);
+    const double Z = (double)p0.z + (double)rem_value(p2.y);
     key[i] = (uint32_t)(tr.key_base + cell_of_fast(tr, X, Y, Z));
 }
 
@@ -737,8 +737,7 @@ template <bool SPIN>
 __device__ __forceinline__ void move_load(const MoveParams &mp, const FusedPtrs &A,
                                           const TrapDev &tr, uint32_t j, MoveAtom &m)
 {
-    rec_load_stream(A.r, j, m.f);
-    m.id = ld_stream_at(A.id, j);
+    rec_load_stream(A.r, j, m.f, m.id);
     m.sx = m.sy = m.sz = 0.0f;
     if (SPIN) { m.sx = ld_at(A.s[0], j); m.sy = ld_at(A.s[1], j); m.sz = ld_at(A.s[2], j); }
     m.kold = 0;
@@ -761,6 +760,25 @@ __device__ __forceinline__ void rotate_about(const F3 &n, float phi, float &sx, 
     sz = fmaf(sz, cs, fmaf(kz, sn, n.z * om));
 }
 
+// Half a velocity kick, v += a dt / 2 (Kahan), with the trap's acceleration
+// a at the position; in the quadrupole a = g (x, y, 4 z) with g = -mu B' /
+// (m rho), so each component is one fma into the remainder.
+__device__ __forceinline__ void half_kick(const TrapDev &tr, FState &f, float hdt)
+{
+    if (tr.kind == kTrapQuad) {
+        const float r2 = quad_r2(f.px, f.py, f.pz);
+        const float g = r2 > 0.0f ? (tr.f_neg_mu_over_m * tr.f_dBdz * hdt) * rsqrt_f(r2) : 0.0f;
+        kahan_fma(f.vx, f.rvx, g, f.px);
+        kahan_fma(f.vy, f.rvy, g, f.py);
+        kahan_fma(f.vz, f.rvz, g, 4.0f * f.pz);
+    } else {
+        const F3 a = accel_f(tr, f.px, f.py, f.pz);
+        kahan_fma(f.vx, f.rvx, a.x, hdt);
+        kahan_fma(f.vy, f.rvy, a.y, hdt);
+        kahan_fma(f.vz, f.rvz, a.z, hdt);
+    }
+}
+
 // Spin substeps, Kahan velocity Verlet, wrap, evaporation test, next key;
 // stores the atom into slot i of B.  vals = (vx^2, vy^2, vz^2, U) of a
 // survivor; lost_e = energy of a removed atom.
@@ -774,10 +792,7 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
 {
     const float dt = mp.f_dt;
     const float hdt = 0.5f * dt;
-    F3 a0 = accel_f(tr, m.f.px, m.f.py, m.f.pz);
-    kahan_add(m.f.vx, m.f.rvx, a0.x * hdt);
-    kahan_add(m.f.vy, m.f.rvy, a0.y * hdt);
-    kahan_add(m.f.vz, m.f.rvz, a0.z * hdt);
+    half_kick(tr, m.f, hdt);
 
     if (SPIN) {
         const double dts = mp.dt / mp.substeps;
@@ -811,9 +826,9 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
         m.sx = (float)((double)sx / L); m.sy = (float)((double)sy / L); m.sz = (float)((double)sz / L);
     }
 
-    kahan_add(m.f.px, m.f.rpx, m.f.vx * dt);
-    kahan_add(m.f.py, m.f.rpy, m.f.vy * dt);
-    kahan_add(m.f.pz, m.f.rpz, m.f.vz * dt);
+    kahan_fma(m.f.px, m.f.rpx, m.f.vx, dt);
+    kahan_fma(m.f.py, m.f.rpy, m.f.vy, dt);
+    kahan_fma(m.f.pz, m.f.rpz, m.f.vz, dt);
     if (tr.periodic) {  // wrap the FP64 value, then split it again
         double X = (double)m.f.px + (double)m.f.rpx, Y = (double)m.f.py + (double)m.f.rpy,
                Z = (double)m.f.pz + (double)m.f.rpz;
@@ -821,10 +836,11 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
         if (Y >= tr.hw[1] || Y < -tr.hw[1]) { Y = wrap_periodic(Y, tr.hw[1]); split_df(Y, m.f.py, m.f.rpy); }
         if (Z >= tr.hw[2] || Z < -tr.hw[2]) { Z = wrap_periodic(Z, tr.hw[2]); split_df(Z, m.f.pz, m.f.rpz); }
     }
-    F3 a1 = accel_f(tr, m.f.px, m.f.py, m.f.pz);
-    kahan_add(m.f.vx, m.f.rvx, a1.x * hdt);
-    kahan_add(m.f.vy, m.f.rvy, a1.y * hdt);
-    kahan_add(m.f.vz, m.f.rvz, a1.z * hdt);
+    // the position remainders as they are stored (rounded below the label's
+    // bits), so that the force, the energies and the key are those of the
+    // stored position
+    m.f.rpx = rem_round(m.f.rpx); m.f.rpy = rem_round(m.f.rpy); m.f.rpz = rem_round(m.f.rpz);
+    half_kick(tr, m.f, hdt);
 
     // Per-atom energies, cut test and cell key in FP32 (the FP64 sums and the
     // exact index near a cell face aside): the squared speeds and the
@@ -834,8 +850,10 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
     const float ekf = mp.f_half_mass * (v2x + v2y + v2z);
     double ep;
     if (tr.kind == kTrapQuad) {
-        const float r2 = m.f.px * m.f.px + m.f.py * m.f.py + 4.0f * (m.f.pz * m.f.pz);
-        ep = (double)(tr.f_mu_dBdz * sqrtf(r2));
+        // rho = r2 / sqrt(r2) with the reciprocal square root of the force
+        // (the same r2, so one rsq serves both)
+        const float r2 = quad_r2(m.f.px, m.f.py, m.f.pz);
+        ep = r2 > 0.0f ? (double)(tr.f_mu_dBdz * (r2 * rsqrt_f(r2))) : 0.0;
     } else if (tr.kind == kTrapIP) {
         ep = potential_energy(tr, (double)m.f.px + (double)m.f.rpx, (double)m.f.py + (double)m.f.rpy,
                               (double)m.f.pz + (double)m.f.rpz);
@@ -883,11 +901,10 @@ __device__ __forceinline__ void move_update(const MoveParams &mp, const TrapDev 
         if (mp.count_moved) moved = kout != m.kold;
     }
     const uint32_t u = (uint32_t)i;
-    rec_store_stream(B.r, u, m.f);
+    rec_store_stream(B.r, u, m.f, m.id);
     if (SPIN) {
         st_stream_at(B.s[0], u, m.sx); st_stream_at(B.s[1], u, m.sy); st_stream_at(B.s[2], u, m.sz);
     }
-    st_stream_at(B.id, u, m.id);
     key_out[u] = kout;   // cached: the re-sort reads the keys next
 }
 
@@ -1164,7 +1181,7 @@ void k_move_fused(
                 const f4 v1 = __builtin_nontemporal_load(q + kRecTile);
                 const f4 v2 = __builtin_nontemporal_load(q + 2 * kRecTile);
                 p0[k] = __builtin_nontemporal_load(q);
-                at[k].id = ld_stream_at(A.id, jj[k]);
+                at[k].id = rec_label(p0[k].w, v2.x, v2.y);
                 at[k].sx = at[k].sy = at[k].sz = 0.0f;
                 if (SPIN) {
                     at[k].sx = ld_at(A.s[0], jj[k]);
@@ -1219,9 +1236,10 @@ void k_move_fused(
             if (at[k].t >= 0) {
                 const f4 v = L.v[kExt + l];
                 const f2 w = L.w[kExt + l];
-                at[k].f.px = p0[k].x; at[k].f.py = p0[k].y; at[k].f.pz = p0[k].z; at[k].f.rpx = p0[k].w;
+                at[k].f.px = p0[k].x; at[k].f.py = p0[k].y; at[k].f.pz = p0[k].z;
+                at[k].f.rpx = rem_value(p0[k].w);
                 at[k].f.vx = v.x; at[k].f.vy = v.y; at[k].f.vz = v.z; at[k].f.rvx = v.w;
-                at[k].f.rpy = rpy[k]; at[k].f.rpz = rpz[k];
+                at[k].f.rpy = rem_value(rpy[k]); at[k].f.rpz = rem_value(rpz[k]);
                 at[k].f.rvy = w.x; at[k].f.rvz = w.y;
             }
         }
@@ -1626,7 +1644,6 @@ __global__ __launch_bounds__(kFBlock) void k_gather_fused(FusedPtrs dst, FusedPt
     if (spin)
 #pragma unroll
         for (int c = 0; c < 3; ++c) dst.s[c][i] = src.s[c][j];
-    dst.id[i] = src.id[j];
 }
 
 // ---------------------------------------------------------------------------
@@ -1902,7 +1919,7 @@ void launch_obs_finalize(int n_traps, int max_trap_atoms, int move_block, double
 void launch_gather_fused(const FusedPtrs &dst, const FusedPtrs &src, const uint32_t *map,
                          const int *n_dev, int n_cap, int spin, hipStream_t st, Profiler *prof)
 {
-    DSMC_LAUNCH(prof, "gather_fused", (double)n_cap * (4.0 + 2.0 * (52 + (spin ? 12 : 0))),
+    DSMC_LAUNCH(prof, "gather_fused", (double)n_cap * (4.0 + 2.0 * (48 + (spin ? 12 : 0))),
                 k_gather_fused, fgrid(n_cap, kFBlock), kFBlock, st, dst, src, map, n_dev, spin);
 }
 

@@ -6,6 +6,7 @@
 #include "dsmc_device.hpp"
 #include "dsmc_internal.hpp"
 #include "init_device.hpp"
+#include "label_bits.hpp"
 #include "plain.hpp"
 
 namespace dsmc {
@@ -39,50 +40,59 @@ __host__ __device__ __forceinline__ size_t rec_index(uint32_t j)
 
 // Device pointers of one FP32 state buffer.
 struct FusedPtrs {
-    f4    *r;        // per-slot state records (layout above)
+    f4    *r;        // per-slot state records (layout above), with the labels
     float *s[3];     // Bloch spin vector (spin runs only)
-    int   *id;       // atom label of each slot
 };
 
-__device__ __forceinline__ void rec_load(const f4 *r, uint32_t j, FState &a)
+__device__ __forceinline__ void rec_unpack(const f4 &p0, const f4 &p1, const f4 &p2, FState &a)
+{
+    a.px = p0.x; a.py = p0.y; a.pz = p0.z; a.rpx = rem_value(p0.w);
+    a.vx = p1.x; a.vy = p1.y; a.vz = p1.z; a.rvx = p1.w;
+    a.rpy = rem_value(p2.x); a.rpz = rem_value(p2.y); a.rvy = p2.z; a.rvz = p2.w;
+}
+
+// the state of slot j (the position remainders' values), and its label
+__device__ __forceinline__ void rec_load(const f4 *r, uint32_t j, FState &a, int *label = nullptr)
 {
     const f4 *q = r + rec_index(j);
     const f4 p0 = q[0], p1 = q[kRecTile], p2 = q[2 * kRecTile];
-    a.px = p0.x; a.py = p0.y; a.pz = p0.z; a.rpx = p0.w;
-    a.vx = p1.x; a.vy = p1.y; a.vz = p1.z; a.rvx = p1.w;
-    a.rpy = p2.x; a.rpz = p2.y; a.rvy = p2.z; a.rvz = p2.w;
+    rec_unpack(p0, p1, p2, a);
+    if (label) *label = rec_label(p0.w, p2.x, p2.y);
 }
 
 // Non-temporal: for the move's streaming read of the whole state (it is the
 // last reader of the source buffer in a step; the cache is left to the
 // re-sort's and the collide's data).
-__device__ __forceinline__ void rec_load_stream(const f4 *r, uint32_t j, FState &a)
+__device__ __forceinline__ void rec_load_stream(const f4 *r, uint32_t j, FState &a, int &label)
 {
     const f4 *q = r + rec_index(j);
     const f4 p0 = __builtin_nontemporal_load(q), p1 = __builtin_nontemporal_load(q + kRecTile),
              p2 = __builtin_nontemporal_load(q + 2 * kRecTile);
-    a.px = p0.x; a.py = p0.y; a.pz = p0.z; a.rpx = p0.w;
-    a.vx = p1.x; a.vy = p1.y; a.vz = p1.z; a.rvx = p1.w;
-    a.rpy = p2.x; a.rpz = p2.y; a.rvy = p2.z; a.rvz = p2.w;
+    rec_unpack(p0, p1, p2, a);
+    label = rec_label(p0.w, p2.x, p2.y);
 }
 
 // The move's write of the whole state: the position and remainder planes
 // non-temporal (only the next move reads them all), the velocity plane
 // through the cache (the next step's collide reads it for its trial pairs).
-__device__ __forceinline__ void rec_store_stream(f4 *r, uint32_t j, const FState &a)
+// The position remainders are already rounded (rem_round).
+__device__ __forceinline__ void rec_store_stream(f4 *r, uint32_t j, const FState &a, int label)
 {
     f4 *q = r + rec_index(j);
-    __builtin_nontemporal_store(f4{ a.px, a.py, a.pz, a.rpx }, q);
+    __builtin_nontemporal_store(f4{ a.px, a.py, a.pz, rem_put(a.rpx, label_part(label, 0)) }, q);
     q[kRecTile] = f4{ a.vx, a.vy, a.vz, a.rvx };
-    __builtin_nontemporal_store(f4{ a.rpy, a.rpz, a.rvy, a.rvz }, q + 2 * kRecTile);
+    __builtin_nontemporal_store(f4{ rem_put(a.rpy, label_part(label, 1)),
+                                    rem_put(a.rpz, label_part(label, 2)), a.rvy, a.rvz },
+                                q + 2 * kRecTile);
 }
 
-__device__ __forceinline__ void rec_store(f4 *r, uint32_t j, const FState &a)
+__device__ __forceinline__ void rec_store(f4 *r, uint32_t j, const FState &a, int label)
 {
     f4 *q = r + rec_index(j);
-    q[0] = f4{ a.px, a.py, a.pz, a.rpx };
+    q[0] = f4{ a.px, a.py, a.pz, rem_tag(a.rpx, label_part(label, 0)) };
     q[kRecTile] = f4{ a.vx, a.vy, a.vz, a.rvx };
-    q[2 * kRecTile] = f4{ a.rpy, a.rpz, a.rvy, a.rvz };
+    q[2 * kRecTile] = f4{ rem_tag(a.rpy, label_part(label, 1)), rem_tag(a.rpz, label_part(label, 2)),
+                          a.rvy, a.rvz };
 }
 
 // Observable accumulators per trap (doubles).

@@ -13,10 +13,12 @@
 
 #include <cmath>
 
+#ifndef DSMC_HOST_DEVICE
 #if defined(__HIPCC__)
 #define DSMC_HOST_DEVICE __host__ __device__
 #else
 #define DSMC_HOST_DEVICE
+#endif
 #endif
 
 namespace dsmc {

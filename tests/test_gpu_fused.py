@@ -38,6 +38,14 @@ def test_fused_index_sort_table_exact(orc):
         sim.set_state(ref.s["pos"], ref.s["vel"])
         pos = sim.copy_out(_abi.FIELD_POS)  # float + remainder, as FP64
         keys = sim.copy_out(_abi.FIELD_CELL_ID)
+        # the labels travel in the position remainders, which keep 13
+        # significant bits: positions to 2^-37 relative, velocities exact
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_ATOM_ID), np.arange(run.n_atoms))
+        assert np.all(np.abs(pos - ref.s["pos"]) <= 2.0 ** -36 * np.abs(ref.s["pos"]))
+        np.testing.assert_array_equal(sim.copy_out(_abi.FIELD_VEL),
+                                      ref.s["vel"].astype(np.float32).astype(np.float64)
+                                      + (ref.s["vel"] - ref.s["vel"].astype(np.float32))
+                                      .astype(np.float32))
         expect = orc.index_atoms(ref.p, pos, np.arange(run.n_atoms, dtype=np.int32))
         np.testing.assert_array_equal(keys, expect)
         sim.sort_arrays()

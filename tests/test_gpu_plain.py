@@ -169,7 +169,9 @@ def test_init_samplers_match_oracle(orc, kind, precision):
     Maxwell velocities) and the quadrupole; in both numerical forms (the
     FP32 form stores the FP64 sample as value + remainder, and its spin as
     the Bloch vector of the aligned spinor).  To 1e-13 (the device log, exp
-    and sin differ from glibc in the last ulp)."""
+    and sin differ from glibc in the last ulp); the FP32 form's positions to
+    2^-36, the precision of value + remainder when the remainder also holds
+    the atom's label (13 significant bits)."""
     n = 20000
     if kind == "ip":
         run = configs.reference(Nc=40, N=n).with_(mcwf=False, spin_loss=False)
@@ -194,7 +196,8 @@ def test_init_samplers_match_oracle(orc, kind, precision):
     assert np.array_equal(np.sort(ids), np.arange(n))
     p = orc.params_from(run.with_(precision="fp64"))
     st = orc.init_distribution(p, n, t.T, None if not any(t.T_axis) else t.T_axis, spin=run.spin)
-    np.testing.assert_allclose(pos, st["pos"], rtol=1e-13, atol=1e-20)
+    np.testing.assert_allclose(pos, st["pos"], rtol=2.0 ** -36 if precision == "fp32" else 1e-13,
+                               atol=1e-20)
     np.testing.assert_allclose(vel, st["vel"], rtol=1e-13, atol=1e-18)
     if run.spin:
         if precision == "fp64":
