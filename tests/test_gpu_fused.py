@@ -447,7 +447,9 @@ def test_reads_after_a_removing_step():
 def test_energy_cut_survivors(orc):
     """One step with a fixed cut: survivors = atoms whose FP64 energy after the
     move is below eta kB T (oracle move of the same state), away from the
-    threshold by more than the FP32 trajectory error."""
+    threshold by more than the FP32 trajectory error; each removed atom's
+    evaporation record, stored at its label (which the state carries in its
+    position remainders), holds its position and velocity after that move."""
     run = quad(50000, 40, sigma=0.0, energy_cut=True)
     T = run.traps[0].T
     run.traps[0].eta_start = run.traps[0].eta_end = 5.0
@@ -459,6 +461,8 @@ def test_energy_cut_survivors(orc):
         sim.step(1)  # the second sort removes the atoms tagged by the first move
         alive = set(sim.copy_out(_abi.FIELD_ATOM_ID).tolist())
         nev = int(sim.obs_log()["n_evap"].sum())
+        ev_pos = sim.copy_out(_abi.FIELD_EVAP_POS).reshape(-1, 3)
+        ev_vel = sim.copy_out(_abi.FIELD_EVAP_VEL).reshape(-1, 3)
     ref.step(1)
     p = ref.p
     E = np.array([0.0])
@@ -473,6 +477,10 @@ def test_energy_cut_survivors(orc):
     assert expect_alive <= alive
     assert not (expect_dead & alive)
     assert nev >= len(expect_dead)
+    dead = np.array(sorted(expect_dead))
+    assert len(dead) > 100
+    np.testing.assert_allclose(ev_pos[dead], ref.s["pos"][dead], rtol=1e-6, atol=1e-10)
+    np.testing.assert_allclose(ev_vel[dead], ref.s["vel"][dead], rtol=1e-6, atol=1e-7)
     del p
 
 
