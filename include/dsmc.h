@@ -152,7 +152,8 @@ enum {
     DSMC_FIELD_SIGVRMAX,       /* n_keys double                                    */
     DSMC_FIELD_COLLISION_COUNT,/* C+1 int (FP64 form)                              */
     DSMC_FIELD_TRAP_START,     /* n_traps + 1 int                                  */
-    DSMC_FIELD_POS_COMP,       /* n x 3 double: Kahan remainders (FP32 form)       */
+    DSMC_FIELD_POS_COMP,       /* n x 3 double: Kahan remainders (FP32 form), */
+                               /* kept to 13 significant bits                      */
     DSMC_FIELD_VEL_COMP,       /* n x 3 double: Kahan remainders (FP32 form)       */
     DSMC_FIELD_OBS,            /* steps x n_traps dsmc_observables (whole log)     */
     DSMC_FIELD_SPIN_UP,        /* n int: 1 spin up, 0 spin down (atomIsSpinUp)     */
