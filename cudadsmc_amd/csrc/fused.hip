@@ -1717,9 +1717,10 @@ void launch_collide_fused(const TrapDev *traps, const int *key_base, int n_traps
 
 double move_fused_bytes_per_atom(int spin)
 {
-    // read: perm 4 + 12 floats + id 4 (+ 3 spin floats); write: same + key 4
-    const double state = 12 * 4 + 4 + (spin ? 12 : 0);
-    return 4.0 + state + state + 4.0;
+    // read: perm 4 + the 12 floats of the state records, which carry the label
+    // (+ 3 spin floats) + the last sort's key 4; write: the same state + key 4
+    const double state = 12 * 4 + (spin ? 12 : 0);
+    return 4.0 + state + 4.0 + state + 4.0;
 }
 
 template <int BS, int APT, bool COLL>
