@@ -414,7 +414,8 @@ def main():
                     help="trap groups of the batched workload, one context and stream each, "
                          "stepped together (1: one context)")
     ap.add_argument("--no-graph", action="store_true",
-                    help="launch the step's kernels one by one instead of replaying a HIP graph")
+                    help="launch the step's kernels one by one instead of replaying a HIP graph "
+                         "(every leg)")
     ap.add_argument("--tune-cells", action="store_true",
                     help="also time the batched step at each cells-per-trap target of the "
                          "cells autotuner and report its choice (the measured workload keeps "
@@ -500,13 +501,15 @@ def main():
         result["drift"] = drift_leg(configs.batched_traps())
         log(f"[bench] batched 1e4 steps: max |dE/E| {result['drift']['max_abs_dE_over_E']:.2e}")
     if "fp64" in legs:
-        run64 = configs.quadrupole_1e6("fp64").with_(obs_capacity=3 * args.single_steps + args.warmup + 64)
+        run64 = configs.quadrupole_1e6("fp64").with_(obs_capacity=3 * args.single_steps + args.warmup + 64,
+                                                     use_graph=not args.no_graph)
         sim, r = run_gpu_leg(run64, args.single_steps, args.warmup)
         sim.close()
         log(f"[bench] FP64 plain N=1e6: {r['value']:.4e} atom-steps/s")
         result["fp64_plain"] = r
     if "fp32" in legs:
-        run32 = configs.quadrupole_1e6("fp32").with_(obs_capacity=3 * args.single_steps + args.warmup + 64)
+        run32 = configs.quadrupole_1e6("fp32").with_(obs_capacity=3 * args.single_steps + args.warmup + 64,
+                                                     use_graph=not args.no_graph)
         sim, r = run_gpu_leg(run32, args.single_steps, args.warmup)
         r["energy_drift"] = energy_drift(sim, args.single_steps)
         sim.close()
@@ -518,7 +521,7 @@ def main():
     if "spin" in legs:
         spin_steps = 200
         run_s = configs.spin_evaporation("fp32").with_(
-            obs_capacity=3 * spin_steps + args.warmup + 64 + 50)
+            obs_capacity=3 * spin_steps + args.warmup + 64 + 50, use_graph=not args.no_graph)
         # the energy cut ramps over 2000 steps here instead of 2e5, so that
         # atoms leave during the timed steps
         run_s.traps[0].eta_steps = 2000

@@ -1730,9 +1730,11 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
                               { 64, 2 }, { 128, 2 }, { 256, 2 }, { 512, 2 } };
     double best = 1e30, pref = 1e30;
     int best_bs = ctx->block, best_apt = ctx->apt;
-    // 256 threads x 2 atoms measured best in the step on MI355X; the trial
-    // moves cannot reproduce the step's cache state, so another candidate
-    // must beat it by more than 2 % to be taken
+    // 256 threads x 2 atoms measured best in the step on MI355X (batched,
+    // single-trap and spin workloads); the trial moves cannot reproduce the
+    // step's cache state, nor do they see the per-block work that smaller
+    // blocks add to the finalize and the re-sort, so another candidate must
+    // beat it by more than 5 % to be taken
     const int pref_bs = 256, pref_apt = 2;
     hipEvent_t b, e;
     DSMC_HIP(hipEventCreate(&b));
@@ -1766,7 +1768,7 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
         if (f < best) { best = f; best_bs = bs; best_apt = apt; }
         if (bs == pref_bs && apt == pref_apt) pref = f;
     }
-    if (pref <= 1.02 * best) { best_bs = pref_bs; best_apt = pref_apt; }
+    if (pref <= 1.05 * best) { best_bs = pref_bs; best_apt = pref_apt; }
     ctx->prof.enabled = was;
     (void)hipEventDestroy(b);
     (void)hipEventDestroy(e);
