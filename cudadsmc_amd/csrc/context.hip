@@ -1725,16 +1725,19 @@ dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out)
         mp.count_moved = 1;
         mp.key_prev = ctx->sorted_key;
     }
-    // (threads per block, atoms per thread)
-    const int cands[9][2] = { { 64, 1 }, { 128, 1 }, { 256, 1 }, { 512, 1 }, { 1024, 1 },
-                              { 64, 2 }, { 128, 2 }, { 256, 2 }, { 512, 2 } };
+    // (threads per block, atoms per thread): shapes of at least 512 atoms per
+    // block.  Smaller blocks multiply the per-block work that trial moves do
+    // not weigh (each move block's trap table and sums, the finalize's
+    // records, the re-sort's counts): on MI355X a 128 x 2 move made the
+    // batched step 26 % slower, and a 64 x 2 move that won the trials made
+    // the spin step 2 % slower.
+    const int cands[4][2] = { { 512, 1 }, { 1024, 1 }, { 256, 2 }, { 512, 2 } };
     double best = 1e30, pref = 1e30;
     int best_bs = ctx->block, best_apt = ctx->apt;
     // 256 threads x 2 atoms measured best in the step on MI355X (batched,
     // single-trap and spin workloads); the trial moves cannot reproduce the
-    // step's cache state, nor do they see the per-block work that smaller
-    // blocks add to the finalize and the re-sort, so another candidate must
-    // beat it by more than 5 % to be taken
+    // step's cache state, so another candidate must beat it by more than 5 %
+    // to be taken
     const int pref_bs = 256, pref_apt = 2;
     hipEvent_t b, e;
     DSMC_HIP(hipEventCreate(&b));

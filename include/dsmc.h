@@ -270,10 +270,10 @@ typedef struct dsmc_kernel_stat {
 } dsmc_kernel_stat;
 dsmc_status dsmc_profile_read(dsmc_ctx *ctx, dsmc_kernel_stat *stats, int max_stats,
                               int *n_stats);
-/* FP32 form: choose the fused move's block size and atoms per thread by
- * timing trial moves on this context's current state (their results are
- * discarded); 256 threads x 2 atoms is kept unless a trial beats it by more
- * than 5 %.  The next sort after a tune on sorted state is a full one.
+/* FP32 form: choose the fused move's block size and atoms per thread (shapes
+ * of at least 512 atoms per block) by timing trial moves on this context's
+ * current state (their results are discarded); 256 threads x 2 atoms is kept
+ * unless a trial beats it by more than 5 %.  The next sort after a tune on sorted state is a full one.
  * block_size_out (may be NULL) receives the block size; the FP64 form keeps
  * 256.  No reference counterpart (the reference's block sizes are fixed). */
 dsmc_status dsmc_autotune(dsmc_ctx *ctx, int *block_size_out);
