@@ -579,7 +579,11 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     const uint64_t lt = lanemask_lt();
     // the chunk's keys, the one before it and the 64 after it (for the end
     // of its last cell) are loaded with the alive count, the step and the
-    // trap table; kNoKey marks slots past the alive atoms
+    // trap table's first 2 kFBlock entries; kNoKey marks slots past the alive
+    // atoms
+    const int tb_j = (int)threadIdx.x;
+    const int tb0 = tb_j < T + 1 ? key_base[tb_j] : 0;
+    const int tb1 = tb_j + kFBlock < T + 1 ? key_base[tb_j + kFBlock] : 0;
     uint32_t kk[kCollItems];
 #pragma unroll
     for (int it = 0; it < kCollItems; ++it) {
@@ -598,7 +602,10 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     if (jp >= n) pk = kNoKey;
     if (chunk0 > n) k_before = kNoKey;
     const uint32_t k_after = __shfl(pk, 0, kWave);
-    load_table(kb, key_base, T + 1);
+    if (tb_j < T + 1) kb[tb_j] = tb0;
+    if (tb_j + kFBlock < T + 1) kb[tb_j + kFBlock] = tb1;
+    for (int j = tb_j + 2 * kFBlock; j < T + 1; j += kFBlock) kb[j] = key_base[j];
+    __syncthreads();
     int nh = 0, nm = 0;
     uint32_t klast = 0;   // key of the chunk's last cell head
 #pragma unroll
