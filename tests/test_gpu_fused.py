@@ -247,7 +247,7 @@ def test_fused_step_equals_split(case, monkeypatch):
     an LDS copy of each wave's velocities and its neighbours', oversized cells
     across span boundaries in an edge pass) against the collide and move
     kernels launched one after the other: bitwise equal states, labels,
-    collision counts and removals.  "batched": three traps with evaporation,
+    collision counts, removals and the removed atoms' records.  "batched": three traps with evaporation,
     graph replay and a block-size autotune half way; "dense": cells of tens to
     hundreds of atoms (many cross span boundaries, many run wave-parallel),
     with 64- and 256-thread blocks."""
@@ -271,10 +271,11 @@ def test_fused_step_equals_split(case, monkeypatch):
             sim.step(steps[1])
             lg = sim.obs_log()
             out[split] = [sim.copy_out(f) for f in (_abi.FIELD_POS, _abi.FIELD_VEL, _abi.FIELD_ATOM_ID)]
-            out[split] += [lg["collisions"].copy(), lg["n_evap"].copy(), lg["ek"].copy()]
+            out[split] += [lg["collisions"].copy(), lg["n_evap"].copy(), lg["ek"].copy(),
+                           sim.copy_out(_abi.FIELD_EVAP_POS), sim.copy_out(_abi.FIELD_EVAP_VEL)]
     assert int(out["1"][3].sum()) > 100
-    for a, b in zip(out["1"][:5], out["0"][:5]):
-        np.testing.assert_array_equal(a, b)
+    for a, b in zip(out["1"][:5] + out["1"][6:], out["0"][:5] + out["0"][6:]):
+        np.testing.assert_array_equal(a, b)   # the removed atoms' records at their labels too
     # the FP64 energy sums are per move block: the autotuner may choose
     # another block size for the fused kernel, which changes only the order
     # of the additions
