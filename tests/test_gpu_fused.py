@@ -396,6 +396,30 @@ def test_batched_trap_equals_standalone():
     assert int(log["n_evap"].sum()) > 0
 
 
+def test_host_reads_between_graph_replays():
+    """A host read between two runs of graph-replayed steps leaves the run
+    bitwise as it is without the read (host reads change no device state that
+    later steps use)."""
+    run = _batch_run()
+    out = {}
+    for mode in ("plain", "read"):
+        print(f"host reads between replays: {mode}", flush=True)
+        with Simulation(run) as sim:
+            sim.init_distribution()
+            if mode == "read":
+                sim.step(30)
+                _ = sim.copy_out(_abi.FIELD_POS)
+                sim.step(30)
+            else:
+                sim.step(60)
+            out[mode] = [sim.copy_out(f) for f in (_abi.FIELD_ATOM_ID, _abi.FIELD_POS,
+                                                   _abi.FIELD_VEL, _abi.FIELD_CELL_ID)]
+            out[mode].append(sim.obs_log()["n_evap"].copy())
+    assert int(out["plain"][4].sum()) > 0
+    for a, b in zip(out["plain"], out["read"]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_reads_after_a_removing_step():
     """Host reads right after a step whose move removed atoms in several traps:
     the removed atoms keep their slots (dead key) until the next sort, and
