@@ -247,7 +247,8 @@ static TrapDev make_trapdev(const dsmc_config &c, const dsmc_trap &t, int index,
     d.B0 = t.B0; d.dBdx = t.dBdx; d.d2Bdx2 = t.d2Bdx2; d.dBdz = t.dBdz;
     d.mu = t.mu;
     d.neg_mu_over_m = (-t.mu) / c.mass;
-    d.alpha = t.n_real / (double)t.n_atoms;
+    // an empty trap (allowed in a batch) has no cell to collide in
+    d.alpha = t.n_atoms > 0 ? t.n_real / (double)t.n_atoms : 0.0;
     d.coll_c1 = d.alpha * c.lpc * c.dt / d.cell_vol;
     d.coll_c1s = d.coll_c1 * c.sigma;
     double Tm = t.T;

@@ -1591,9 +1591,12 @@ __global__ __launch_bounds__(kObsBlock) void k_obs_sums(const TrapDev *__restric
     __shared__ double red[(kObsBlock / kWave) * 6];
     const int t = blockIdx.x;
     const TrapDev &tr = traps[t];
+    // a trap without alive atoms keeps first = INT_MAX, last = -1: no slots
+    // (a + threadIdx.x would overflow there, so the range is tested first)
     const int a = first[t], b = last[t];
+    const int i0 = a <= b ? a + (int)threadIdx.x : b + 1;
     double v[6] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };   // v2x, v2y, v2z, U, n, (unused)
-    for (int i = a + (int)threadIdx.x; i <= b; i += kObsBlock) {
+    for (int i = i0; i <= b; i += kObsBlock) {
         if (key[i] == dead_key) continue;
         FState f;
         rec_load(s.r, (uint32_t)i, f);

@@ -60,7 +60,7 @@ enum { DSMC_SORT_INCREMENTAL = 0, DSMC_SORT_FULL = 1 };
  * plus the CLI arguments "Nc N" of main.cu:45-69. */
 typedef struct dsmc_trap {
     int    kind;            /* DSMC_TRAP_*                                        */
-    int    n_atoms;         /* test particles                                     */
+    int    n_atoms;         /* test particles (0: an empty trap in a batch)       */
     int    nc[3];           /* cells per dimension (cellsPerDimension)            */
     int    philox_stream;   /* added to the trap's index to give its Philox       */
                             /* stream: a trap's random numbers depend only on the */

@@ -471,6 +471,64 @@ def test_reads_after_a_removing_step():
             assert E.max() <= cut * (1 + 1e-6)
 
 
+def test_empty_and_emptied_traps():
+    """A batch with traps of no atoms (first, middle and last) and a trap whose
+    cut (0.05 kB T) removes every atom in its first step: their slot ranges
+    stay empty, their log records and dsmc_observe sums read zero, and the
+    populated traps are bitwise the same as when run alone."""
+    spec = [(0, 2.0, 20e-6, 0.0), (3000, 1.5, 30e-6, 6.0), (0, 2.0, 20e-6, 0.0),
+            (2000, 2.5, 15e-6, 0.05), (20000, 4.0, 8e-6, 6.0), (0, 2.0, 20e-6, 0.0)]
+    traps = []
+    for n, dBdz, T, eta in spec:
+        t = configs.quadrupole_trap(n, configs.cells_for_trap(n), T=T, dBdz=dBdz, n_real=3e6)
+        t.eta_start = t.eta_end = eta
+        t.T_cut = T
+        traps.append(t)
+    run = configs.Run(traps=traps, precision="fp32", energy_cut=True, dt=1e-6, seed=17,
+                      name="empty-traps")
+    steps = 30
+    print("empty and emptied traps: batch", flush=True)
+    with Simulation(run) as sim:
+        sim.init_distribution()
+        sim.step(steps)
+        ts = sim.copy_out(_abi.FIELD_TRAP_START)
+        ids = sim.copy_out(_abi.FIELD_ATOM_ID)
+        pos = sim.copy_out(_abi.FIELD_POS)
+        vel = sim.copy_out(_abi.FIELD_VEL)
+        log = sim.obs_log()
+        obs = sim.observe()
+        n_alive = sim.info().n_alive
+    assert ts[0] == ts[1] == 0 and ts[2] == ts[3] == ts[4]
+    assert ts[5] == ts[6] == n_alive == len(ids) == len(np.unique(ids))
+    for t in (0, 2, 5):
+        for f in ("n_alive", "n_evap", "collisions", "ek", "ep"):
+            assert not log[f][:, t].any(), (t, f)
+        assert (obs[t].n_alive, obs[t].ek, obs[t].ep) == (0, 0.0, 0.0)
+    assert int(log["n_evap"][:, 3].sum()) == spec[3][0]
+    assert log["n_alive"][0, 3] == 0 and obs[3].n_alive == 0 and obs[3].ek == 0.0
+    begin = np.cumsum([0] + [t.n_atoms for t in traps])
+    total_coll = 0
+    for t in (1, 4):
+        tr = traps[t]
+        tr.philox_stream = t
+        print(f"empty and emptied traps: trap {t} alone", flush=True)
+        solo = configs.Run(traps=[tr], precision="fp32", energy_cut=True, dt=1e-6, seed=17)
+        with Simulation(solo) as s1:
+            s1.init_distribution()
+            s1.step(steps)
+            np.testing.assert_array_equal(ids[ts[t]:ts[t + 1]] - begin[t],
+                                          s1.copy_out(_abi.FIELD_ATOM_ID))
+            np.testing.assert_array_equal(pos[ts[t]:ts[t + 1]], s1.copy_out(_abi.FIELD_POS))
+            np.testing.assert_array_equal(vel[ts[t]:ts[t + 1]], s1.copy_out(_abi.FIELD_VEL))
+            l1 = s1.obs_log()
+        np.testing.assert_array_equal(log["collisions"][:, t], l1["collisions"][:, 0])
+        np.testing.assert_array_equal(log["n_evap"][:, t], l1["n_evap"][:, 0])
+        np.testing.assert_array_equal(log["n_alive"][:, t], l1["n_alive"][:, 0])
+        np.testing.assert_allclose(log["ek"][:, t], l1["ek"][:, 0], rtol=1e-12)
+        total_coll += int(l1["collisions"].sum())
+    assert total_coll > 0
+
+
 def test_energy_cut_survivors(orc):
     """One step with a fixed cut: survivors = atoms whose FP64 energy after the
     move is below eta kB T (oracle move of the same state), away from the
