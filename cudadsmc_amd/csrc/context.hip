@@ -1027,7 +1027,8 @@ static void copy_field(dsmc_ctx *x, int field, void *host, size_t bytes)
     const int n_slots = x->fp32 ? fused_live(x, live) : host_slots(x);
     const int n = x->fp32 ? (int)live.size() : n_slots;
     const size_t need = field_size(x, field, n);
-    if (need == 0) fail(DSMC_ENOTSUP, "field not available in this form");
+    // a field of this form has a size at one atom; at no alive atoms it is empty
+    if (field_size(x, field, 1) == 0) fail(DSMC_ENOTSUP, "field not available in this form");
     if (bytes < need) fail(DSMC_EINVAL, "host buffer too small");
     const size_t N = (size_t)n;
     double *hd = (double *)host;

@@ -529,6 +529,35 @@ def test_empty_and_emptied_traps():
     assert total_coll > 0
 
 
+def test_every_atom_removed():
+    """A cut of 0.05 kB T removes every atom of a single trap in its first
+    step; the steps after it run on no atoms: the log reads zero alive atoms
+    from the first record on, the removed energy equals the initial energy
+    (the step conserves it), every atom has its evaporation record, reads
+    return empty arrays, and dsmc_observe and dsmc_calculate_temp read zero."""
+    run = quad(20000, 30, energy_cut=True)
+    tr = run.traps[0]
+    tr.eta_start = tr.eta_end = 0.05
+    tr.T_cut = tr.T
+    print("every atom removed", flush=True)
+    with Simulation(run) as sim:
+        sim.init_distribution()
+        e0 = sim.observe()[0]
+        sim.step(21)
+        log = sim.obs_log()
+        obs = sim.observe()[0]
+        assert sim.info().n_alive == 0 and sim.calculate_temp() == 0.0
+        assert sim.copy_out(_abi.FIELD_POS).shape == (0, 3)
+        assert sim.copy_out(_abi.FIELD_ATOM_ID).size == 0
+        ev = sim.copy_out(_abi.FIELD_EVAP_POS).reshape(-1, 3)
+    assert not log["n_alive"][:, 0].any()
+    assert int(log["n_evap"][0, 0]) == run.n_atoms and not log["n_evap"][1:, 0].any()
+    assert not log["collisions"][1:, 0].any() and not log["ek"][1:, 0].any()
+    np.testing.assert_allclose(log["e_evap"][0, 0], e0.ek + e0.ep, rtol=1e-4)
+    assert ev.shape == (run.n_atoms, 3) and np.isfinite(ev).all()
+    assert (obs.n_alive, obs.ek, obs.ep) == (0, 0.0, 0.0)
+
+
 def test_energy_cut_survivors(orc):
     """One step with a fixed cut: survivors = atoms whose FP64 energy after the
     move is below eta kB T (oracle move of the same state), away from the
