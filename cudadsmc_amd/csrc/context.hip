@@ -1029,6 +1029,7 @@ static void copy_field(dsmc_ctx *x, int field, void *host, size_t bytes)
     const size_t need = field_size(x, field, n);
     // a field of this form has a size at one atom; at no alive atoms it is empty
     if (field_size(x, field, 1) == 0) fail(DSMC_ENOTSUP, "field not available in this form");
+    if (need == 0) return;
     if (bytes < need) fail(DSMC_EINVAL, "host buffer too small");
     const size_t N = (size_t)n;
     double *hd = (double *)host;

@@ -529,17 +529,19 @@ def test_empty_and_emptied_traps():
     assert total_coll > 0
 
 
-def test_every_atom_removed():
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_every_atom_removed(precision):
     """A cut of 0.05 kB T removes every atom of a single trap in its first
     step; the steps after it run on no atoms: the log reads zero alive atoms
     from the first record on, the removed energy equals the initial energy
     (the step conserves it), every atom has its evaporation record, reads
-    return empty arrays, and dsmc_observe and dsmc_calculate_temp read zero."""
-    run = quad(20000, 30, energy_cut=True)
+    return empty arrays, and dsmc_observe and dsmc_calculate_temp read zero.
+    Both numerical forms."""
+    run = quad(20000, 30, precision=precision, energy_cut=True)
     tr = run.traps[0]
     tr.eta_start = tr.eta_end = 0.05
     tr.T_cut = tr.T
-    print("every atom removed", flush=True)
+    print(f"every atom removed, {precision}", flush=True)
     with Simulation(run) as sim:
         sim.init_distribution()
         e0 = sim.observe()[0]
