@@ -213,6 +213,32 @@ def test_incremental_sort_is_the_stable_sort():
             sim.move_atoms()
 
 
+@pytest.mark.parametrize("n", [(1 << 24) - 1, 1 << 24])
+def test_sort_at_the_incremental_limit(n):
+    """The incremental re-sort at its largest size (2^24 - 1 slots: 8192
+    tiles of 2048, the locate's LDS table of tile samples full) and one slot
+    beyond it (the step's sort then falls back to the radix sort): after some
+    steps the sort equals numpy's stable argsort of the keys, and the log's
+    energy stays within 1e-5 of the initial energy."""
+    run = quad(n, 128)
+    print(f"sort at the incremental limit: n = {n}", flush=True)
+    with Simulation(run) as sim:
+        sim.init_distribution()
+        e0 = sim.observe()[0]
+        sim.step(6)
+        keys = sim.copy_out(_abi.FIELD_CELL_ID).astype(np.int64)
+        sim.sort_arrays()
+        perm = sim.copy_out(_abi.FIELD_PERM)
+        srt = sim.copy_out(_abi.FIELD_CELL_ID)
+        log = sim.obs_log()
+    assert len(keys) == n and int(np.count_nonzero(np.diff(keys) < 0)) > 0
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(perm, order)
+    np.testing.assert_array_equal(srt, keys[order])
+    E0 = e0.ek + e0.ep
+    assert np.max(np.abs(log["ek"][:, 0] + log["ep"][:, 0] - E0)) / E0 < 1e-5
+
+
 def test_incremental_sort_equals_full_sort(monkeypatch):
     """Whole batched runs (evaporation on, graph replay, a block-size autotune
     half way) with the incremental and the full sort are bitwise identical,
