@@ -9,14 +9,21 @@
 // compaction and the alive count never has to visit the host.
 //
 // Step (one motion step, lpc = 1):
-//   sort(keys)           -> sorted keys + permutation        (primitives.hip)
-//   k_table_fused        -> cell end per occupied key, trap ranges, alive count
-//   k_collide_fused      -> one thread per occupied cell, state read through
-//                           the permutation (no reorder pass)
+//   sort(keys)           -> sorted keys + permutation: incrementally from the
+//                           last move's mover masks (k_ins_locate, k_ins_merge
+//                           in primitives.hip) or by a full radix sort, after
+//                           which k_table_fused finds the trap ranges and the
+//                           alive count
+//   k_collide_fused      -> each wave scans 512 sorted slots for cells of two
+//                           or more atoms and runs their trials, velocities
+//                           read through the permutation (no reorder pass)
 //   k_move_fused         -> gathers state through the permutation into the
 //                           other buffer (so the state ends up sorted), spin
 //                           substeps, Kahan-compensated Verlet, evaporation,
-//                           next cell key, FP64 observables per trap
+//                           next cell key, FP64 observables per block
+//   k_obs_finalize       -> per-trap sums in fixed order and the log record,
+//                           the next trap ranges and alive count, and the
+//                           incremental re-sort's first phase
 // Every kernel reads the alive count and the step number from device memory,
 // so the loop can be captured once and replayed as a HIP graph.
 #include "fused.hpp"
@@ -559,8 +566,10 @@ __device__ __forceinline__ int run_cell(const CollideParams &cp, const CellJob &
 // `wave_trials` trials are then run by the whole wave.  When the chunk lies
 // in one trap, its parameters are wave-uniform (scalar loads).  Five waves
 // per SIMD: the kernel is bound by the latency of its dependent loads (key,
-// sigma v_r,max, permutation, velocities).
-__global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
+// sigma v_r,max, permutation, velocities).  CB threads per block (one or
+// four waves).
+template <int CB>
+__global__ __launch_bounds__(CB, 5) void k_collide_fused(
     const TrapDev *__restrict__ traps, const int *__restrict__ key_base, int T, CollideParams cp,
     const uint32_t *__restrict__ key, const uint32_t *__restrict__ perm,
     const int *__restrict__ n_dev, int n_cap, FusedPtrs s, float *__restrict__ sigvrmax,
@@ -569,21 +578,21 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
 {
     constexpr int kChunk = kWave * kCollItems;
     extern __shared__ int kb[];
-    __shared__ int heads[kFBlock / kWave][kChunk + 1];
-    __shared__ uint32_t hkey[kFBlock / kWave][kChunk];   // key of each head
-    __shared__ short multi[kFBlock / kWave][kChunk];   // heads index of cells with >= 2 atoms
-    __shared__ short heavy[kFBlock / kWave][kChunk];
+    __shared__ int heads[CB / kWave][kChunk + 1];
+    __shared__ uint32_t hkey[CB / kWave][kChunk];   // key of each head
+    __shared__ short multi[CB / kWave][kChunk];   // heads index of cells with >= 2 atoms
+    __shared__ short heavy[CB / kWave][kChunk];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    const int chunk0 = (blockIdx.x * (kFBlock / kWave) + w) * kChunk;
+    const int chunk0 = (blockIdx.x * (CB / kWave) + w) * kChunk;
     const VelGlobal vg = { s.r, perm };
     const uint64_t lt = lanemask_lt();
     // the chunk's keys, the one before it and the 64 after it (for the end
     // of its last cell) are loaded with the alive count, the step and the
-    // trap table's first 2 kFBlock entries; kNoKey marks slots past the alive
+    // trap table's first 2 CB entries; kNoKey marks slots past the alive
     // atoms
     const int tb_j = (int)threadIdx.x;
     const int tb0 = tb_j < T + 1 ? key_base[tb_j] : 0;
-    const int tb1 = tb_j + kFBlock < T + 1 ? key_base[tb_j + kFBlock] : 0;
+    const int tb1 = tb_j + CB < T + 1 ? key_base[tb_j + CB] : 0;
     uint32_t kk[kCollItems];
 #pragma unroll
     for (int it = 0; it < kCollItems; ++it) {
@@ -603,8 +612,8 @@ __global__ __launch_bounds__(kFBlock, 5) void k_collide_fused(
     if (chunk0 > n) k_before = kNoKey;
     const uint32_t k_after = __shfl(pk, 0, kWave);
     if (tb_j < T + 1) kb[tb_j] = tb0;
-    if (tb_j + kFBlock < T + 1) kb[tb_j + kFBlock] = tb1;
-    for (int j = tb_j + 2 * kFBlock; j < T + 1; j += kFBlock) kb[j] = key_base[j];
+    if (tb_j + CB < T + 1) kb[tb_j + CB] = tb1;
+    for (int j = tb_j + 2 * CB; j < T + 1; j += CB) kb[j] = key_base[j];
     __syncthreads();
     int nh = 0, nm = 0;
     uint32_t klast = 0;   // key of the chunk's last cell head
@@ -1718,9 +1727,19 @@ void launch_collide_fused(const TrapDev *traps, const int *key_base, int n_traps
     const char *env = std::getenv("DSMC_COLLIDE_WAVE_TRIALS");
     const int wave_trials = env ? std::atoi(env) : kWaveTrialsDefault;
     if (prof && prof->enabled) prof->begin(st, "collide_fused", (double)n_cap * 4.0);
-    hipLaunchKernelGGL(k_collide_fused, dim3(fgrid(n_cap, kFBlock * kCollItems)), dim3(kFBlock), lds,
-                       st, traps, key_base, n_traps, cp, key, perm, n_dev, n_cap, s, sigvrmax,
-                       coll_accum, step_dev, wave_trials);
+    // One wave per block when the context holds several traps: with trap
+    // groups stepped on two streams, one-wave blocks find room beside the
+    // other group's move blocks (batched step of two groups 157 -> 154 us on
+    // MI355X, bitwise the same results); a single trap's collide runs about
+    // 1 us faster in blocks of four waves.
+    if (n_traps > 1)
+        hipLaunchKernelGGL(k_collide_fused<kWave>, dim3(fgrid(n_cap, kWave * kCollItems)),
+                           dim3(kWave), lds, st, traps, key_base, n_traps, cp, key, perm, n_dev,
+                           n_cap, s, sigvrmax, coll_accum, step_dev, wave_trials);
+    else
+        hipLaunchKernelGGL(k_collide_fused<kFBlock>, dim3(fgrid(n_cap, kFBlock * kCollItems)),
+                           dim3(kFBlock), lds, st, traps, key_base, n_traps, cp, key, perm, n_dev,
+                           n_cap, s, sigvrmax, coll_accum, step_dev, wave_trials);
     DSMC_HIP(hipGetLastError());
     if (prof && prof->enabled) prof->end(st);
 }
