@@ -577,6 +577,11 @@ def test_every_atom_removed(precision):
         assert sim.info().n_alive == 0 and sim.calculate_temp() == 0.0
         assert sim.copy_out(_abi.FIELD_POS).shape == (0, 3)
         assert sim.copy_out(_abi.FIELD_ATOM_ID).size == 0
+        # a field the form does not keep is still refused, not read as empty
+        lacking = _abi.FIELD_ACC if precision == "fp32" else _abi.FIELD_POS_COMP
+        with pytest.raises(_abi.DsmcError) as err:
+            sim.copy_out(lacking)
+        assert err.value.status == _abi.DSMC_ENOTSUP
         ev = sim.copy_out(_abi.FIELD_EVAP_POS).reshape(-1, 3)
     assert not log["n_alive"][:, 0].any()
     assert int(log["n_evap"][0, 0]) == run.n_atoms and not log["n_evap"][1:, 0].any()
