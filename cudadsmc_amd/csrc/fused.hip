@@ -1731,8 +1731,10 @@ void launch_collide_fused(const TrapDev *traps, const int *key_base, int n_traps
     // groups stepped on two streams, one-wave blocks find room beside the
     // other group's move blocks (batched step of two groups 157 -> 154 us on
     // MI355X, bitwise the same results); a single trap's collide runs about
-    // 1 us faster in blocks of four waves.
-    if (n_traps > 1)
+    // 1 us faster in blocks of four waves.  Every block holds the trap table
+    // in LDS, so beyond 511 traps (a 2 KB table) one-wave blocks would put
+    // fewer than five waves on a SIMD, and four-wave blocks are used.
+    if (n_traps > 1 && n_traps + 1 <= 512)
         hipLaunchKernelGGL(k_collide_fused<kWave>, dim3(fgrid(n_cap, kWave * kCollItems)),
                            dim3(kWave), lds, st, traps, key_base, n_traps, cp, key, perm, n_dev,
                            n_cap, s, sigvrmax, coll_accum, step_dev, wave_trials);

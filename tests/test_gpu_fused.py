@@ -555,6 +555,46 @@ def test_empty_and_emptied_traps():
     assert total_coll > 0
 
 
+@pytest.mark.parametrize("n_traps", [600, 8192])
+def test_many_traps(n_traps):
+    """Batches of many small traps (64 atoms, 8^3 cells each): 600 traps,
+    beyond the 511 up to which the collide runs one-wave blocks, and 8192,
+    the most a context takes (every kernel's trap table fills 32 KB of LDS).
+    The trap ranges hold 64 atoms each, and the first, middle and last traps
+    are bitwise the same as when run alone."""
+    traps = [configs.quadrupole_trap(64, 8, T=10e-6 + 20e-6 * (i % 7) / 6, dBdz=2.5, n_real=3e8)
+             for i in range(n_traps)]
+    run = configs.Run(traps=traps, precision="fp32", dt=1e-6, seed=21, obs_capacity=16,
+                      name="many-traps")
+    steps = 12
+    print(f"many traps: {n_traps}", flush=True)
+    with Simulation(run) as sim:
+        sim.init_distribution()
+        sim.step(steps)
+        ts = sim.copy_out(_abi.FIELD_TRAP_START)
+        ids = sim.copy_out(_abi.FIELD_ATOM_ID)
+        pos = sim.copy_out(_abi.FIELD_POS)
+        vel = sim.copy_out(_abi.FIELD_VEL)
+        log = sim.obs_log()
+        n_alive = sim.info().n_alive
+    assert len(ts) == n_traps + 1 and ts[0] == 0 and ts[-1] == n_alive == 64 * n_traps
+    assert np.all(np.diff(ts) == 64)
+    for t in (0, n_traps // 2, n_traps - 1):
+        tr = traps[t]
+        tr.philox_stream = t
+        print(f"many traps: {n_traps}, trap {t} alone", flush=True)
+        solo = configs.Run(traps=[tr], precision="fp32", dt=1e-6, seed=21, obs_capacity=16)
+        with Simulation(solo) as s1:
+            s1.init_distribution()
+            s1.step(steps)
+            np.testing.assert_array_equal(ids[ts[t]:ts[t + 1]] - 64 * t, s1.copy_out(_abi.FIELD_ATOM_ID))
+            np.testing.assert_array_equal(pos[ts[t]:ts[t + 1]], s1.copy_out(_abi.FIELD_POS))
+            np.testing.assert_array_equal(vel[ts[t]:ts[t + 1]], s1.copy_out(_abi.FIELD_VEL))
+            l1 = s1.obs_log()
+        np.testing.assert_array_equal(log["collisions"][:, t], l1["collisions"][:, 0])
+        np.testing.assert_allclose(log["ek"][:, t], l1["ek"][:, 0], rtol=1e-12)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_every_atom_removed(precision):
     """A cut of 0.05 kB T removes every atom of a single trap in its first
