@@ -30,17 +30,18 @@ def test_split_traps_balances_atoms_and_keeps_streams():
 
 
 @pytest.mark.gpu
-def test_group_equals_one_context():
+@pytest.mark.parametrize("groups", [2, 3])
+def test_group_equals_one_context(groups):
     steps = 300
     run = configs.batched_traps(n_traps=24, steps=steps).with_(obs_capacity=steps + 64)
-    print(f"group of 2 contexts vs one: 24 traps, {run.n_atoms} atoms, {steps} steps", flush=True)
+    print(f"group of {groups} contexts vs one: 24 traps, {run.n_atoms} atoms, {steps} steps", flush=True)
     with Simulation(run) as one:
         one.init_distribution()
         one.step(steps)
         lg1 = one.obs_log()
         ts1 = one.copy_out(_abi.FIELD_TRAP_START)
         ref = [one.copy_out(f) for f in (_abi.FIELD_POS, _abi.FIELD_VEL, _abi.FIELD_ATOM_ID)]
-    parts = configs.split_traps(run, 2)
+    parts = configs.split_traps(run, groups)
     with SimulationGroup(parts) as grp:
         grp.init_distribution()
         grp.step(steps, chunk=50)
